@@ -1,0 +1,175 @@
+/* paragraph.h — C ABI of the MI355X-native ParaGraph GPU index-build stage.
+ *
+ * Replaces the reference's GPU stage: the 12 kernel launches on
+ * compute_stream plus the H2D/D2H copies in tests/test_t2iend.cu:850-1155
+ * (kernels in include/Gbuilder.cuh and include/Gsearcher.cuh).  The
+ * reference exposes no library entry point for this stage (its
+ * libGGidxbuildCUDA exports only test(), src/Gbuilder.cu:1-8; its Builder
+ * class is a commented-out stub, include/Builder.hpp:6-41), so each entry
+ * point below names the reference code it stands in for.
+ *
+ * Conventions
+ *  - Plain C types only; ids are uint32_t, vectors float32 row-major.
+ *  - tomb = 0xFFFFFFFF (int32 -1) marks empty slots, as in the reference.
+ *  - Every call returns 0 on success or a negative pg_status; nothing
+ *    calls exit().  pg_last_error() gives the message.
+ *  - A context owns its HIP stream, its workspace and its output staging;
+ *    one context per host thread.  Calls do not spawn CPU threads, so they
+ *    may run beside an OpenMP CPU stage (tests/test_t2iend.cu:888-962).
+ */
+#ifndef PARAGRAPH_H_
+#define PARAGRAPH_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_TOMB 0xFFFFFFFFu
+
+typedef enum pg_status {
+  PG_OK = 0,
+  PG_ERR_INVALID = -1,  /* parameter or size check failed */
+  PG_ERR_OOM = -2,      /* device allocation failed */
+  PG_ERR_HIP = -3,      /* a HIP runtime call or kernel failed */
+  PG_ERR_BUSY = -4,     /* pg_query: work still in flight */
+  PG_ERR_NO_DEVICE = -5 /* no usable GPU */
+} pg_status;
+
+typedef enum pg_precision {
+  PG_FP32 = 0,     /* plain FP32 path: reproduces the reference ids exactly */
+  PG_BF16 = 1,     /* BF16 screening on MFMA, FP32 re-rank before pruning */
+  PG_FP8E4M3 = 2   /* FP8 e4m3 screening with per-vector scales */
+} pg_precision;
+
+/* Build parameters.  The reference fixes these as constexpr template
+ * arguments (tests/test_t2iend.cu:670-727); here they are runtime values.
+ * pg_params_init() fills the mapping used by this project for a given
+ * (n_base, dim, M, L). */
+typedef struct pg_params {
+  uint32_t n_base;      /* N base vectors (< 2^31: the MSB is the expanded flag,
+                           Gsearcher.cuh:906) */
+  uint32_t dim;         /* vector dimension */
+  uint32_t n_query;     /* gt rows (training queries) */
+  uint32_t gt_k;        /* gt row width (128, -1 padded) */
+  uint32_t proj_deg;    /* phase-1 projection degree (16) */
+  uint32_t proj_pr;     /* phase-1 pruned cap (15) */
+  uint32_t proj_rev;    /* phase-1 reverse cap (111) */
+  uint32_t M;           /* link degree = round pruned cap */
+  uint32_t link_rev;    /* round reverse cap (126 - M) */
+  uint32_t Km;          /* search top-list length (beam L) */
+  uint32_t Kp;          /* nodes expanded per iteration */
+  uint32_t Kd;          /* neighbours read per expanded node */
+  uint32_t topk;        /* search output width (2L) */
+  uint32_t reset_iter;  /* visited-table reset period (15) */
+  uint32_t table_size;  /* visited-table slots (power of two) */
+  uint32_t rounds;      /* link rounds (2) */
+  uint32_t entry_seed;  /* entry points (seed + i) mod N (3030517) */
+  uint32_t max_iters;   /* search iteration watchdog (not in the reference) */
+  uint32_t precision;   /* pg_precision */
+  uint32_t rerank;      /* screened paths: FP32 re-rank width (0 = topk) */
+  uint32_t tune_flags;  /* perf-only knobs, 0 = defaults / autotune */
+  uint32_t reserved[7];
+} pg_params;
+
+/* Counters of one build (same meaning as the oracle's). */
+typedef struct pg_stats {
+  uint64_t prune_dist;    /* node->candidate distances in prune steps */
+  uint64_t prune_pairs;   /* occlusion pair distances in prune steps */
+  uint64_t merge_dist;    /* node->candidate distances in merge steps */
+  uint64_t merge_pairs;   /* occlusion pair distances in merge steps */
+  uint64_t search_dist;   /* visited-miss distance evaluations */
+  uint64_t search_iters;  /* search iterations */
+  uint64_t search_capped; /* queries stopped by the watchdog */
+  uint64_t rev_overflow;  /* targets whose reverse list exceeded the cap */
+  uint64_t screen_fallback; /* screened paths: exact re-evaluations */
+  uint64_t reserved[7];
+  float stage_ms[16];     /* per-stage device time of the last build (see
+                             pg_stage_name) */
+  uint32_t n_stages;
+} pg_stats;
+
+typedef struct pg_ctx pg_ctx;
+
+/* Fills *p with the project's mapping of (N, dim, M, L) onto the reference
+ * knobs: link_rev = 126-M, Km = L, Kd = 16, Kp = ceil(L/16), topk = 2L,
+ * reset_iter 15, table 4096, 2 rounds, seed 3030517, projection 16/15/111
+ * (reference values: tests/test_t2iend.cu:676-707). */
+int pg_params_init(pg_params* p, uint32_t n_base, uint32_t dim,
+                   uint32_t n_query, uint32_t M, uint32_t L);
+
+/* Checks a parameter set without touching the GPU (used by pg_create). */
+int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len);
+
+/* Creates a context on `device` and allocates its workspace for *p
+ * (replaces the cudaMalloc block, tests/test_t2iend.cu:781-797).  Returns
+ * NULL on failure; pg_last_error(NULL) then holds the reason. */
+pg_ctx* pg_create(int device, const pg_params* p);
+void pg_destroy(pg_ctx* ctx);
+
+/* Whole GPU stage from host buffers: H2D of base (N x dim) and gt
+ * (n_query x gt_k), the 12-launch pipeline, D2H of the projection
+ * (N x proj_deg) and link (N x M) rows (tests/test_t2iend.cu:858-1155).
+ * Returns after enqueueing; outputs are valid once pg_wait() returns 0. */
+int pg_gpu_stage_async(pg_ctx* ctx, const float* base, const uint32_t* gt,
+                       uint32_t* out_proj, uint32_t* out_link);
+
+/* Same pipeline on device-resident buffers (no copies). */
+int pg_gpu_stage_device_async(pg_ctx* ctx, const float* d_base,
+                              const uint32_t* d_gt, uint32_t* d_proj,
+                              uint32_t* d_link);
+
+int pg_wait(pg_ctx* ctx);   /* blocks until the last enqueued work ends */
+int pg_query(pg_ctx* ctx);  /* PG_OK if done, PG_ERR_BUSY if in flight */
+int pg_get_stats(pg_ctx* ctx, pg_stats* out);  /* waits, then copies */
+int pg_reset_stats(pg_ctx* ctx);
+const char* pg_last_error(const pg_ctx* ctx);
+const char* pg_stage_name(uint32_t i);
+
+/* ---- Single stages on device buffers (for parity tests and callers that
+ * stitch their own pipeline).  All run on the context's stream and use its
+ * n_base / dim / search parameters. ---- */
+
+/* match_top1_kernel_v0 (Gbuilder.cuh:104-135): winner[b] = lowest query
+ * whose gt row starts with b, else tomb. */
+int pg_run_match(pg_ctx* ctx, const uint32_t* d_gt, uint32_t n_query,
+                 uint32_t gt_k, uint32_t* d_winner);
+
+/* fusion_prune_reverse_kernel_v0 (Gbuilder.cuh:1105-1244), pruned part.
+ * Candidates per node come from d_rows (N x D) or, when d_rows is NULL,
+ * from the top-1 row of the winner (gt[w][1..gt_k-2], two tombs; D = gt_k).
+ * Writes pruned (N x mp) and counts (N). */
+int pg_run_prune(pg_ctx* ctx, const float* d_base, const uint32_t* d_rows,
+                 uint32_t D, const uint32_t* d_gt, const uint32_t* d_winner,
+                 uint32_t gt_k, uint32_t mp, uint32_t* d_pruned,
+                 uint32_t* d_pcnt);
+
+/* add_reverse (Gbuilder.cuh:1020-1038) as a CSR by target: d_off (N+1),
+ * d_srcs (sum of counts).  Segment order is unspecified; the merge step
+ * selects the cap smallest sources in ascending order. */
+int pg_run_reverse(pg_ctx* ctx, const uint32_t* d_pruned,
+                   const uint32_t* d_pcnt, uint32_t mp, uint32_t* d_off,
+                   uint32_t* d_srcs);
+
+/* merge_and_sort + fusion_merge_sort_prune_kernel (Gbuilder.cuh:1433-1601).
+ * Writes N x deg rows. */
+int pg_run_merge(pg_ctx* ctx, const float* d_base, const uint32_t* d_off,
+                 const uint32_t* d_srcs, uint32_t cap,
+                 const uint32_t* d_pruned, const uint32_t* d_pcnt,
+                 uint32_t mp, uint32_t deg, int strict, uint32_t* d_out);
+
+/* link_process_v0_store_base_data (Gsearcher.cuh:896-1066), generalised
+ * to queries that are not base rows (the unused search_shared_hashtable,
+ * Gsearcher.cuh:108-296, is the reference model for that).  Writes
+ * n_query x topk ids (& 0x7FFFFFFF). */
+int pg_run_search(pg_ctx* ctx, const float* d_queries, uint32_t n_query,
+                  const float* d_base, const uint32_t* d_graph,
+                  uint32_t graph_deg, uint32_t* d_out);
+
+int pg_sync(pg_ctx* ctx);  /* hipStreamSynchronize + error check */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARAGRAPH_H_ */

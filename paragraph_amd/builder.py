@@ -1,0 +1,48 @@
+"""Builder: the reference's (commented-out) Gbuilder::Builder<T> interface.
+
+Reference stub, include/Builder.hpp:6-41:
+    explicit Builder(std::vector<T> const& base, std::vector<id_type> const& gt_ids)
+    template <grid_size, block_size, gt_num, dim> auto LaunchBuild() -> void
+    auto SetGt(std::vector<id_type>&& gt_ids) -> void
+Same constructor arguments and SetGt; LaunchBuild takes the build knobs at
+run time (the reference bakes them into template arguments,
+tests/test_t2iend.cu:670-727) and returns the two arrays the reference copies
+back to the host: the projection rows (N x 16) and the link rows (N x M).
+"""
+import numpy as np
+
+from . import _native
+
+
+class Builder:
+    def __init__(self, base, gt_ids, dim=None):
+        base = np.ascontiguousarray(base, dtype=np.float32)
+        if base.ndim == 1:
+            if dim is None:
+                raise ValueError("flat base needs dim")
+            base = base.reshape(-1, dim)
+        self.base_ = base
+        self.gt_ids_ = None
+        self.SetGt(gt_ids)
+        self.stats_ = None
+
+    def SetGt(self, gt_ids, gt_k=128):
+        g = np.asarray(gt_ids)
+        if g.ndim == 1:
+            g = g.reshape(-1, gt_k)
+        self.gt_ids_ = np.ascontiguousarray(g.astype(np.int64) & 0xFFFFFFFF,
+                                            dtype=np.uint32)
+
+    def LaunchBuild(self, M=32, L=128, device=0, precision=_native.PG_FP32,
+                    **overrides):
+        n, dim = self.base_.shape
+        q, gt_k = self.gt_ids_.shape
+        p = _native.make_params(n, dim, q, M, L, gt_k=gt_k,
+                                precision=precision, **overrides)
+        proj = np.empty((n, p.proj_deg), np.uint32)
+        link = np.empty((n, p.M), np.uint32)
+        with _native.Context(p, device) as ctx:
+            ctx.stage_host(self.base_, self.gt_ids_, proj, link)
+            ctx.wait()
+            self.stats_ = ctx.stats().as_dict()
+        return proj, link

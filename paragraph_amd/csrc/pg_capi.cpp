@@ -1,0 +1,500 @@
+// C ABI of the index-build stage (declared in include/paragraph.h).
+//
+// Host orchestration that replaces the reference's GPU-stage driver
+// (tests/test_t2iend.cu:781-1155): one context owns the stream, the
+// workspace and the timing events, and enqueues the pipeline
+//   match -> prune(top-1 rows) -> reverse -> merge(16, non-strict)
+//   then per round: search -> prune(topk) -> reverse -> merge(M, strict)
+// Buffers the reference reuses by reinterpret_cast (d_space_128_xx/yy) are
+// separate, typed allocations here.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/paragraph.h"
+#include "pg_kernels.h"
+
+namespace {
+
+constexpr int kMaxStages = 16;
+constexpr uint64_t kStatCount = 256ull * 16ull;
+thread_local char g_err[512] = "";
+
+const char* const kStageNames[kMaxStages] = {
+    "match",         "prune_proj",     "reverse_proj",  "merge_proj",
+    "search_round1", "prune_round1",   "reverse_round1", "merge_round1",
+    "search_round2", "prune_round2",   "reverse_round2", "merge_round2",
+    "search_round3", "prune_round3",   "reverse_round3", "merge_round3"};
+
+}  // namespace
+
+struct pg_ctx {
+  int device = 0;
+  pg_params p{};
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  hipEvent_t ev[kMaxStages + 1] = {};
+  int n_stage_events = 0;
+  uint32_t mp_max = 0;
+  // workspace
+  uint32_t* winner = nullptr;
+  uint32_t* pruned = nullptr;
+  uint32_t* pcnt = nullptr;
+  uint32_t* deg = nullptr;
+  uint32_t* cursor = nullptr;
+  uint32_t* off = nullptr;
+  uint32_t* scan_tmp = nullptr;
+  uint32_t* srcs = nullptr;
+  uint32_t* res = nullptr;
+  unsigned long long* stats = nullptr;
+  // staging for the host-pointer entry point (allocated on first use)
+  float* h_base_dev = nullptr;
+  uint32_t* h_gt_dev = nullptr;
+  uint32_t* h_proj_dev = nullptr;
+  uint32_t* h_link_dev = nullptr;
+  char err[512] = "";
+};
+
+namespace {
+
+int set_err(pg_ctx* c, int code, const char* fmt, ...) {
+  char* dst = c ? c->err : g_err;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(dst, 512, fmt, ap);
+  va_end(ap);
+  if (c) std::memcpy(g_err, c->err, sizeof(g_err));
+  return code;
+}
+
+#define PG_HIP(ctx, call)                                                 \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess)                                                 \
+      return set_err((ctx), PG_ERR_HIP, "%s failed: %s", #call,           \
+                     hipGetErrorString(e_));                              \
+  } while (0)
+
+template <typename T>
+int dev_alloc(pg_ctx* c, T** ptr, uint64_t count, const char* what) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(ptr), count * sizeof(T));
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return set_err(c, PG_ERR_OOM, "hipMalloc(%s, %llu bytes): %s", what,
+                   (unsigned long long)(count * sizeof(T)),
+                   hipGetErrorString(e));
+  }
+  return PG_OK;
+}
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+uint32_t next_pow2(uint32_t x) {
+  uint32_t v = 1;
+  while (v < x) v <<= 1;
+  return v;
+}
+
+int record(pg_ctx* c, int idx) {
+  if (idx <= kMaxStages) PG_HIP(c, hipEventRecord(c->ev[idx], c->stream));
+  return PG_OK;
+}
+
+#define PG_TRY(expr)          \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != PG_OK) return rc_; \
+  } while (0)
+
+int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
+                 uint32_t* d_proj, uint32_t* d_link) {
+  const pg_params& p = c->p;
+  const uint32_t N = p.n_base;
+  hipStream_t s = c->stream;
+  int ev = 0;
+  PG_TRY(record(c, ev++));
+  PG_HIP(c, pg::launch_match(d_gt, p.n_query, p.gt_k, N, c->winner, s));
+  PG_TRY(record(c, ev++));
+  PG_HIP(c, pg::launch_prune(d_base, N, p.dim, nullptr, p.gt_k, d_gt,
+                             c->winner, p.gt_k, p.proj_pr, c->pruned, c->pcnt,
+                             c->stats, s));
+  PG_TRY(record(c, ev++));
+  PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.proj_pr, c->deg,
+                               c->off, c->cursor, c->scan_tmp, c->srcs, s));
+  PG_TRY(record(c, ev++));
+  PG_HIP(c, pg::launch_merge(d_base, N, p.dim, c->off, c->srcs, p.proj_rev,
+                             c->pruned, c->pcnt, p.proj_pr, p.proj_deg, 0,
+                             d_proj, c->stats, s));
+  PG_TRY(record(c, ev++));
+  const uint32_t* graph = d_proj;
+  uint32_t gdeg = p.proj_deg;
+  for (uint32_t r = 0; r < p.rounds; ++r) {
+    PG_HIP(c, pg::launch_search(d_base, N, d_base, N, p.dim, graph, gdeg, p.Km,
+                                p.Kp, p.Kd, p.topk, p.reset_iter,
+                                p.entry_seed, p.max_iters, p.table_size,
+                                c->res, c->stats, s));
+    PG_TRY(record(c, ev++));
+    PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, p.topk, nullptr,
+                               nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
+                               s));
+    PG_TRY(record(c, ev++));
+    PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.M, c->deg, c->off,
+                                 c->cursor, c->scan_tmp, c->srcs, s));
+    PG_TRY(record(c, ev++));
+    PG_HIP(c, pg::launch_merge(d_base, N, p.dim, c->off, c->srcs, p.link_rev,
+                               c->pruned, c->pcnt, p.M, p.M, 1, d_link,
+                               c->stats, s));
+    PG_TRY(record(c, ev++));
+    graph = d_link;
+    gdeg = p.M;
+  }
+  c->n_stage_events = ev;
+  PG_HIP(c, hipEventRecord(c->done, s));
+  return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_params_init(pg_params* p, uint32_t n_base, uint32_t dim,
+                   uint32_t n_query, uint32_t M, uint32_t L) {
+  if (!p) return PG_ERR_INVALID;
+  std::memset(p, 0, sizeof(*p));
+  p->n_base = n_base;
+  p->dim = dim;
+  p->n_query = n_query;
+  p->gt_k = 128;
+  p->proj_deg = 16;
+  p->proj_pr = 15;
+  p->proj_rev = 111;
+  p->M = M;
+  p->link_rev = M < 126 ? 126 - M : 0;
+  p->Km = L;
+  p->Kd = 16;
+  p->Kp = (L + 15) / 16;
+  p->topk = 2 * L;
+  p->reset_iter = 15;
+  p->table_size = 4096;
+  p->rounds = 2;
+  p->entry_seed = 3030517;
+  p->max_iters = 100000;
+  p->precision = PG_FP32;
+  p->rerank = 0;
+  return PG_OK;
+}
+
+int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len) {
+  char buf[256] = "";
+  auto fail = [&](const char* why) {
+    std::snprintf(buf, sizeof(buf), "%s", why);
+    if (msg && msg_len) std::snprintf(msg, msg_len, "%s", buf);
+    return PG_ERR_INVALID;
+  };
+  if (!p) return fail("params is NULL");
+  if (p->n_base == 0 || p->n_base >= 0x80000000u)
+    return fail("n_base must be in [1, 2^31) (the id MSB is the expanded flag)");
+  if (p->dim == 0 || p->dim > 1024) return fail("dim must be in [1, 1024]");
+  if (p->gt_k < 3 || p->gt_k > 512) return fail("gt_k must be in [3, 512]");
+  if (p->proj_pr < 2 || p->proj_pr > 128) return fail("proj_pr must be in [2, 128]");
+  if (p->proj_deg < 1 || p->proj_deg > 128) return fail("proj_deg must be in [1, 128]");
+  if (p->proj_rev + p->proj_pr > 128)
+    return fail("proj_rev + proj_pr must be <= 128 (merge list width)");
+  if (p->M < 2 || p->M > 128) return fail("M must be in [2, 128]");
+  if (p->link_rev + p->M > 128) return fail("link_rev + M must be <= 128");
+  if (p->Km == 0 || p->Kp == 0 || p->Kd == 0) return fail("Km, Kp, Kd must be > 0");
+  if (p->Kd > p->proj_deg || (p->rounds > 1 && p->Kd > p->M))
+    return fail("Kd must not exceed the degree of the graph it scans");
+  if (p->topk == 0 || p->topk > p->Km + p->Kp * p->Kd)
+    return fail("topk must be in [1, Km + Kp*Kd]");
+  if (p->topk > 512) return fail("topk must be <= 512");
+  if (p->Km + p->Kp * p->Kd > 4096) return fail("Km + Kp*Kd must be <= 4096");
+  if (p->reset_iter == 0) return fail("reset_iter must be > 0");
+  if (p->table_size == 0 || (p->table_size & (p->table_size - 1)) != 0)
+    return fail("table_size must be a power of two");
+  if (uint64_t(p->table_size) <=
+      uint64_t(p->Km) + uint64_t(p->reset_iter) * p->Kp * p->Kd)
+    return fail("table_size must exceed Km + reset_iter*Kp*Kd (else the "
+                "visited table can fill up)");
+  {
+    uint32_t lp2 = 64;
+    while (lp2 < p->Km + p->Kp * p->Kd) lp2 <<= 1;
+    if (pg::search_smem_bytes(p->table_size, lp2, p->Kp * p->Kd, p->Kp) >
+        160u * 1024u)
+      return fail("search LDS footprint exceeds 160 KiB");
+  }
+  if (p->rounds > 3) return fail("rounds must be <= 3");
+  if (p->max_iters == 0) return fail("max_iters must be > 0");
+  if (p->precision != PG_FP32)
+    return fail("only the FP32 precision path is available in this build");
+  if (msg && msg_len) msg[0] = 0;
+  return PG_OK;
+}
+
+pg_ctx* pg_create(int device, const pg_params* p) {
+  char msg[256];
+  if (pg_params_check(p, msg, sizeof(msg)) != PG_OK) {
+    set_err(nullptr, PG_ERR_INVALID, "invalid params: %s", msg);
+    return nullptr;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_err(nullptr, PG_ERR_NO_DEVICE, "no HIP device available");
+    return nullptr;
+  }
+  if (device < 0 || device >= ndev) {
+    set_err(nullptr, PG_ERR_INVALID, "device %d out of range (%d devices)",
+            device, ndev);
+    return nullptr;
+  }
+  pg_ctx* c = new (std::nothrow) pg_ctx();
+  if (!c) {
+    set_err(nullptr, PG_ERR_OOM, "host allocation failed");
+    return nullptr;
+  }
+  c->device = device;
+  c->p = *p;
+  c->mp_max = p->proj_pr > p->M ? p->proj_pr : p->M;
+  auto fail = [&](int) {
+    std::memcpy(g_err, c->err, sizeof(g_err));
+    pg_destroy(c);
+    return static_cast<pg_ctx*>(nullptr);
+  };
+  if (hipSetDevice(device) != hipSuccess) {
+    set_err(c, PG_ERR_HIP, "hipSetDevice(%d) failed", device);
+    return fail(0);
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->done) != hipSuccess) {
+    set_err(c, PG_ERR_HIP, "stream/event creation failed");
+    return fail(0);
+  }
+  for (int i = 0; i <= kMaxStages; ++i)
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+      set_err(c, PG_ERR_HIP, "event creation failed");
+      return fail(0);
+    }
+  const uint64_t N = p->n_base;
+  const uint32_t nb = (p->n_base + 1023) / 1024 + 1;
+  if (dev_alloc(c, &c->winner, N, "winner") ||
+      dev_alloc(c, &c->pruned, N * c->mp_max, "pruned") ||
+      dev_alloc(c, &c->pcnt, N, "pcnt") || dev_alloc(c, &c->deg, N, "deg") ||
+      dev_alloc(c, &c->cursor, N, "cursor") ||
+      dev_alloc(c, &c->off, N + 1, "off") ||
+      dev_alloc(c, &c->scan_tmp, nb, "scan_tmp") ||
+      dev_alloc(c, &c->srcs, N * c->mp_max, "srcs") ||
+      dev_alloc(c, &c->res, N * p->topk, "search_out") ||
+      dev_alloc(c, &c->stats, kStatCount, "stats"))
+    return fail(0);
+  if (pg::launch_fill_u64(c->stats, kStatCount, 0ull, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    set_err(c, PG_ERR_HIP, "initial clear failed");
+    return fail(0);
+  }
+  return c;
+}
+
+void pg_destroy(pg_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dev_free(c->winner);
+  dev_free(c->pruned);
+  dev_free(c->pcnt);
+  dev_free(c->deg);
+  dev_free(c->cursor);
+  dev_free(c->off);
+  dev_free(c->scan_tmp);
+  dev_free(c->srcs);
+  dev_free(c->res);
+  dev_free(c->stats);
+  dev_free(c->h_base_dev);
+  dev_free(c->h_gt_dev);
+  dev_free(c->h_proj_dev);
+  dev_free(c->h_link_dev);
+  for (int i = 0; i <= kMaxStages; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  if (c->done) (void)hipEventDestroy(c->done);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int pg_gpu_stage_device_async(pg_ctx* c, const float* d_base,
+                              const uint32_t* d_gt, uint32_t* d_proj,
+                              uint32_t* d_link) {
+  if (!c) return set_err(nullptr, PG_ERR_INVALID, "ctx is NULL");
+  if (!d_base || !d_gt || !d_proj || !d_link)
+    return set_err(c, PG_ERR_INVALID, "NULL buffer");
+  PG_HIP(c, hipSetDevice(c->device));
+  return run_pipeline(c, d_base, d_gt, d_proj, d_link);
+}
+
+int pg_gpu_stage_async(pg_ctx* c, const float* base, const uint32_t* gt,
+                       uint32_t* out_proj, uint32_t* out_link) {
+  if (!c) return set_err(nullptr, PG_ERR_INVALID, "ctx is NULL");
+  if (!base || !gt || !out_proj || !out_link)
+    return set_err(c, PG_ERR_INVALID, "NULL buffer");
+  const pg_params& p = c->p;
+  const uint64_t N = p.n_base;
+  PG_HIP(c, hipSetDevice(c->device));
+  if (!c->h_base_dev) {
+    PG_TRY(dev_alloc(c, &c->h_base_dev, N * p.dim, "base"));
+    PG_TRY(dev_alloc(c, &c->h_gt_dev, uint64_t(p.n_query) * p.gt_k, "gt"));
+    PG_TRY(dev_alloc(c, &c->h_proj_dev, N * p.proj_deg, "proj"));
+    PG_TRY(dev_alloc(c, &c->h_link_dev, N * p.M, "link"));
+  }
+  PG_HIP(c, hipMemcpyAsync(c->h_base_dev, base, N * p.dim * sizeof(float),
+                           hipMemcpyHostToDevice, c->stream));
+  PG_HIP(c, hipMemcpyAsync(c->h_gt_dev, gt,
+                           uint64_t(p.n_query) * p.gt_k * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, c->stream));
+  PG_TRY(run_pipeline(c, c->h_base_dev, c->h_gt_dev, c->h_proj_dev,
+                      c->h_link_dev));
+  PG_HIP(c, hipMemcpyAsync(out_proj, c->h_proj_dev,
+                           N * p.proj_deg * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(c, hipMemcpyAsync(out_link, c->h_link_dev, N * p.M * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+  PG_HIP(c, hipEventRecord(c->done, c->stream));
+  return PG_OK;
+}
+
+int pg_wait(pg_ctx* c) {
+  if (!c) return set_err(nullptr, PG_ERR_INVALID, "ctx is NULL");
+  PG_HIP(c, hipStreamSynchronize(c->stream));
+  PG_HIP(c, hipGetLastError());
+  return PG_OK;
+}
+
+int pg_sync(pg_ctx* c) { return pg_wait(c); }
+
+int pg_query(pg_ctx* c) {
+  if (!c) return set_err(nullptr, PG_ERR_INVALID, "ctx is NULL");
+  hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipSuccess) return PG_OK;
+  if (e == hipErrorNotReady) return PG_ERR_BUSY;
+  return set_err(c, PG_ERR_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
+}
+
+int pg_reset_stats(pg_ctx* c) {
+  if (!c) return set_err(nullptr, PG_ERR_INVALID, "ctx is NULL");
+  PG_HIP(c, pg::launch_fill_u64(c->stats, kStatCount, 0ull, c->stream));
+  return pg_wait(c);
+}
+
+int pg_get_stats(pg_ctx* c, pg_stats* out) {
+  if (!c || !out) return set_err(c, PG_ERR_INVALID, "NULL argument");
+  PG_TRY(pg_wait(c));
+  static_assert(kStatCount == 4096, "stat layout");
+  unsigned long long host[kStatCount];
+  PG_HIP(c, hipMemcpy(host, c->stats, sizeof(host), hipMemcpyDeviceToHost));
+  uint64_t f[16] = {};
+  for (int s = 0; s < 256; ++s)
+    for (int k = 0; k < 16; ++k) f[k] += host[s * 16 + k];
+  std::memset(out, 0, sizeof(*out));
+  out->prune_dist = f[0];
+  out->prune_pairs = f[1];
+  out->merge_dist = f[2];
+  out->merge_pairs = f[3];
+  out->search_dist = f[4];
+  out->search_iters = f[5];
+  out->search_capped = f[6];
+  out->rev_overflow = f[7];
+  out->screen_fallback = f[8];
+  out->n_stages = c->n_stage_events > 1 ? uint32_t(c->n_stage_events - 1) : 0u;
+  for (uint32_t i = 0; i < out->n_stages && i < 16; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess)
+      out->stage_ms[i] = ms;
+  }
+  return PG_OK;
+}
+
+const char* pg_last_error(const pg_ctx* c) { return c ? c->err : g_err; }
+
+const char* pg_stage_name(uint32_t i) {
+  return i < uint32_t(kMaxStages) ? kStageNames[i] : "";
+}
+
+int pg_run_match(pg_ctx* c, const uint32_t* d_gt, uint32_t n_query,
+                 uint32_t gt_k, uint32_t* d_winner) {
+  if (!c || !d_winner || (n_query && !d_gt))
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  PG_HIP(c, hipSetDevice(c->device));
+  PG_HIP(c, pg::launch_match(d_gt, n_query, gt_k, c->p.n_base, d_winner,
+                             c->stream));
+  return PG_OK;
+}
+
+int pg_run_prune(pg_ctx* c, const float* d_base, const uint32_t* d_rows,
+                 uint32_t D, const uint32_t* d_gt, const uint32_t* d_winner,
+                 uint32_t gt_k, uint32_t mp, uint32_t* d_pruned,
+                 uint32_t* d_pcnt) {
+  if (!c || !d_base || !d_pruned || !d_pcnt)
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  if (!d_rows) {
+    if (!d_gt || !d_winner) return set_err(c, PG_ERR_INVALID, "top-1 mode needs gt and winner");
+    if (gt_k < 3 || gt_k > 512) return set_err(c, PG_ERR_INVALID, "gt_k out of range");
+    D = gt_k;
+  }
+  if (D == 0 || D > 512) return set_err(c, PG_ERR_INVALID, "D must be in [1, 512]");
+  if (mp < 2 || mp > 128) return set_err(c, PG_ERR_INVALID, "mp must be in [2, 128]");
+  PG_HIP(c, hipSetDevice(c->device));
+  PG_HIP(c, pg::launch_prune(d_base, c->p.n_base, c->p.dim, d_rows, D, d_gt,
+                             d_winner, gt_k, mp, d_pruned, d_pcnt, c->stats,
+                             c->stream));
+  return PG_OK;
+}
+
+int pg_run_reverse(pg_ctx* c, const uint32_t* d_pruned, const uint32_t* d_pcnt,
+                   uint32_t mp, uint32_t* d_off, uint32_t* d_srcs) {
+  if (!c || !d_pruned || !d_pcnt || !d_off || !d_srcs)
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  if (mp == 0 || mp > 128) return set_err(c, PG_ERR_INVALID, "mp must be in [1, 128]");
+  PG_HIP(c, hipSetDevice(c->device));
+  PG_HIP(c, pg::launch_reverse(d_pruned, d_pcnt, c->p.n_base, mp, c->deg, d_off,
+                               c->cursor, c->scan_tmp, d_srcs, c->stream));
+  return PG_OK;
+}
+
+int pg_run_merge(pg_ctx* c, const float* d_base, const uint32_t* d_off,
+                 const uint32_t* d_srcs, uint32_t cap,
+                 const uint32_t* d_pruned, const uint32_t* d_pcnt, uint32_t mp,
+                 uint32_t deg, int strict, uint32_t* d_out) {
+  if (!c || !d_base || !d_off || !d_srcs || !d_pruned || !d_pcnt || !d_out)
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  if (cap + mp > 128) return set_err(c, PG_ERR_INVALID, "cap + mp must be <= 128");
+  if (deg == 0 || deg > 128) return set_err(c, PG_ERR_INVALID, "deg must be in [1, 128]");
+  PG_HIP(c, hipSetDevice(c->device));
+  PG_HIP(c, pg::launch_merge(d_base, c->p.n_base, c->p.dim, d_off, d_srcs, cap,
+                             d_pruned, d_pcnt, mp, deg, strict, d_out,
+                             c->stats, c->stream));
+  return PG_OK;
+}
+
+int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
+                  const float* d_base, const uint32_t* d_graph,
+                  uint32_t graph_deg, uint32_t* d_out) {
+  if (!c || !d_base || !d_graph || !d_out || (n_query && !d_queries))
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  const pg_params& p = c->p;
+  if (p.Kd > graph_deg)
+    return set_err(c, PG_ERR_INVALID, "Kd (%u) exceeds graph degree (%u)", p.Kd,
+                   graph_deg);
+  PG_HIP(c, hipSetDevice(c->device));
+  PG_HIP(c, pg::launch_search(d_queries, n_query, d_base, p.n_base, p.dim,
+                              d_graph, graph_deg, p.Km, p.Kp, p.Kd, p.topk,
+                              p.reset_iter, p.entry_seed, p.max_iters,
+                              p.table_size, d_out, c->stats, c->stream));
+  return PG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// Device-side helpers shared by the index-build kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+namespace pg {
+
+constexpr uint32_t kTomb = 0xFFFFFFFFu;
+constexpr uint32_t kMaskedTomb = 0x7FFFFFFFu;  // tomb after & 0x7FFFFFFF
+constexpr uint32_t kMsb = 0x80000000u;
+constexpr int kWave = 64;
+constexpr int kStatSlots = 256;  // spread counter atomics over this many rows
+constexpr int kStatFields = 16;
+constexpr uint64_t kPadKey = ~0ull;
+
+enum StatField {
+  ST_PRUNE_DIST = 0,
+  ST_PRUNE_PAIRS,
+  ST_MERGE_DIST,
+  ST_MERGE_PAIRS,
+  ST_SEARCH_DIST,
+  ST_SEARCH_ITERS,
+  ST_SEARCH_CAPPED,
+  ST_REV_OVERFLOW,
+  ST_SCREEN_FALLBACK,
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Float -> uint32 whose unsigned order equals the float order (no NaNs).
+__device__ __forceinline__ uint32_t ordered_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & kMsb) ? ~u : (u | kMsb);
+}
+__device__ __forceinline__ float from_ordered(uint32_t o) {
+  return __uint_as_float((o & kMsb) ? (o & ~kMsb) : ~o);
+}
+__device__ __forceinline__ uint32_t rotl1(uint32_t x) { return (x << 1) | (x >> 31); }
+__device__ __forceinline__ uint32_t rotr1(uint32_t x) { return (x >> 1) | (x << 31); }
+
+// Search-list key: (distance, id & 0x7FFFFFFF, expanded bit) in one u64.
+__device__ __forceinline__ uint64_t search_key(float d, uint32_t raw_id) {
+  return (uint64_t(ordered_bits(d)) << 32) | rotl1(raw_id);
+}
+__device__ __forceinline__ uint32_t key_raw_id(uint64_t k) {
+  return rotr1(uint32_t(k));
+}
+__device__ __forceinline__ float key_dist(uint64_t k) {
+  return from_ordered(uint32_t(k >> 32));
+}
+
+__device__ __forceinline__ bool is_stop_id(uint32_t id, uint32_t n) {
+  return id == kTomb || id == kMaskedTomb || id >= n;
+}
+
+// Sequential FMA chain over the whole row (Gbuilder.cuh:785-797), negated.
+// `x` is the wave-uniform row, `y` the per-lane row.
+__device__ __forceinline__ float chain_dist(const float* __restrict__ x,
+                                            const float* __restrict__ y,
+                                            uint32_t dim) {
+  float s = 0.0f;
+  if ((dim & 3u) == 0) {
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const float4* y4 = reinterpret_cast<const float4*>(y);
+    const uint32_t n4 = dim >> 2;
+#pragma unroll 4
+    for (uint32_t i = 0; i < n4; ++i) {
+      float4 a = x4[i], b = y4[i];
+      s = __builtin_fmaf(a.x, b.x, s);
+      s = __builtin_fmaf(a.y, b.y, s);
+      s = __builtin_fmaf(a.z, b.z, s);
+      s = __builtin_fmaf(a.w, b.w, s);
+    }
+  } else {
+    for (uint32_t i = 0; i < dim; ++i) s = __builtin_fmaf(x[i], y[i], s);
+  }
+  return -s;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (1ull << lane_id()) - 1ull;
+}
+
+// Ascending bitonic sort of n (power of two, >= 2) keys in LDS by one wave
+// (the workgroup is exactly one wave, so __syncthreads orders LDS traffic).
+template <typename K>
+__device__ void bitonic_sort(K* a, uint32_t n) {
+  const uint32_t lane = lane_id();
+  const uint32_t half = n >> 1;
+  for (uint32_t k = 2; k <= n; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t t = lane; t < half; t += kWave) {
+        uint32_t i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+        uint32_t p = i | j;
+        K x = a[i], y = a[p];
+        bool asc = (i & k) == 0;
+        if ((x > y) == asc) {
+          a[i] = y;
+          a[p] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ void stat_add(unsigned long long* stats, int field,
+                                         unsigned long long v) {
+  if (stats != nullptr && v != 0)
+    atomicAdd(&stats[(blockIdx.x % kStatSlots) * kStatFields + field], v);
+}
+
+}  // namespace pg

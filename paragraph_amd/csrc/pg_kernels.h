@@ -1,0 +1,40 @@
+// Host-side launchers for the index-build kernels (pg_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace pg {
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n,
+                           unsigned long long v, hipStream_t s);
+hipError_t launch_match(const uint32_t* gt, uint32_t n_query, uint32_t gt_k,
+                        uint32_t n_base, uint32_t* winner, hipStream_t s);
+hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
+                        const uint32_t* rows, uint32_t D, const uint32_t* gt,
+                        const uint32_t* winner, uint32_t gt_k, uint32_t mp,
+                        uint32_t* pruned, uint32_t* pcnt,
+                        unsigned long long* stats, hipStream_t s);
+hipError_t launch_reverse(const uint32_t* pruned, const uint32_t* pcnt,
+                          uint32_t n_base, uint32_t mp, uint32_t* deg,
+                          uint32_t* off, uint32_t* cursor, uint32_t* scan_tmp,
+                          uint32_t* srcs, hipStream_t s);
+hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
+                        const uint32_t* off, const uint32_t* srcs,
+                        uint32_t cap, const uint32_t* pruned,
+                        const uint32_t* pcnt, uint32_t mp, uint32_t deg,
+                        int strict, uint32_t* out, unsigned long long* stats,
+                        hipStream_t s);
+size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
+                         uint32_t Kp);
+hipError_t launch_search(const float* queries, uint32_t n_query,
+                         const float* base, uint32_t n_base, uint32_t dim,
+                         const uint32_t* graph, uint32_t gdeg, uint32_t Km,
+                         uint32_t Kp, uint32_t Kd, uint32_t topk,
+                         uint32_t reset_iter, uint32_t entry_seed,
+                         uint32_t max_iters, uint32_t table_size,
+                         uint32_t* out, unsigned long long* stats,
+                         hipStream_t s);
+
+}  // namespace pg

@@ -1,0 +1,707 @@
+// Index-build kernels for MI355X (gfx950, wave64), plain FP32 path.
+//
+// One wavefront (= one 64-lane workgroup) per node or per query.  Distances
+// are computed with explicit FMAs in exactly the reference's summation
+// orders, so neighbour ids match the CPU oracle bit for bit:
+//   prune / merge: sequential chain over the row  (Gbuilder.cuh:785-797)
+//   search:        32 lane-strided chains + 16/8/4/2/1 tree
+//                                     (Gsearcher.cuh:958-976, 1023-1035)
+// Ordering rules where the reference depends on scheduling (C1-C3 in the
+// oracle's header) are implemented deterministically here.
+#include "pg_device.hpp"
+#include "pg_kernels.h"
+
+namespace pg {
+
+// ---------------------------------------------------------------- utilities
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n,
+                                uint32_t v) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* __restrict__ p, uint64_t n,
+                                unsigned long long v) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// ------------------------------------------------------------ K0: top-1 match
+// match_top1_kernel_v0 (Gbuilder.cuh:104-135).  The reference lets an
+// atomicCAS pick the owner of a contested base row; here the lowest query id
+// wins (rule C1) via atomicMin on a per-base winner array.  The row itself
+// is never materialised: the prune step reads gt[winner][1..gt_k-2].
+__global__ void match_winner_kernel(const uint32_t* __restrict__ gt,
+                                    uint32_t n_query, uint32_t gt_k,
+                                    uint32_t n_base,
+                                    uint32_t* __restrict__ winner) {
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n_query;
+       q += gridDim.x * blockDim.x) {
+    uint32_t b = gt[size_t(q) * gt_k];
+    if (b < n_base) atomicMin(&winner[b], q);
+  }
+}
+
+// ------------------------------------------------- RNG occlusion (K2 and K3)
+// Strict RNG over a sorted list held in LDS (Gbuilder.cuh:1150-1198 and
+// :1500-1553).  `skey[i]` = (ordered distance << 32 | original position),
+// `sid[pos]` = id.  kept[0..nk) already holds the first kept entry.  Each
+// 64-candidate block is resolved with lane = candidate: every lane first
+// tests its candidate against the entries kept before the block, then the
+// block is walked in order, each newly kept entry testing the later lanes.
+// Every candidate therefore compares against kept entries in insertion
+// order and stops at the first occluder, as the reference does; the pair
+// counter only counts comparisons the reference would make.
+template <bool kStopIds>
+__device__ uint32_t rng_select(const float* __restrict__ base, uint32_t n_base,
+                               uint32_t dim, const uint64_t* skey,
+                               const uint32_t* sid, uint32_t n_scan,
+                               uint32_t cap, uint32_t* kept, uint32_t nk,
+                               unsigned long long* pair_total) {
+  const uint32_t lane = lane_id();
+  unsigned long long pairs = 0;
+  bool stop = false;
+  for (uint32_t blk = 1; blk < n_scan && !stop && nk < cap; blk += kWave) {
+    const uint32_t pos = blk + lane;
+    const bool in = pos < n_scan;
+    uint32_t my_id = kTomb;
+    float my_d = FLT_MAX;
+    if (in) {
+      uint64_t k = skey[pos];
+      my_id = sid[uint32_t(k)];
+      my_d = from_ordered(uint32_t(k >> 32));
+    }
+    uint32_t lim = kWave;
+    if (kStopIds) {
+      uint64_t sm = __ballot(in && is_stop_id(my_id, n_base));
+      if (sm) {
+        lim = __ffsll((long long)sm) - 1;
+        stop = true;
+      }
+    }
+    const bool active = in && lane < lim;
+    bool occl = false;
+    uint32_t my_pairs = 0;
+    const float* own = base + size_t(active ? my_id : 0) * dim;
+    for (uint32_t k = 0; k < nk; ++k) {
+      const uint32_t kid = kept[k];
+      if (active && !occl) {
+        float pd = chain_dist(base + size_t(kid) * dim, own, dim);
+        ++my_pairs;
+        occl = pd < my_d;
+      }
+    }
+    int last = -1;
+    int cut = kWave;
+    while (true) {
+      uint64_t m = __ballot(active && !occl && int(lane) > last);
+      if (m == 0) break;
+      const int j = __ffsll((long long)m) - 1;
+      const uint32_t kid = __shfl(my_id, j, 64);
+      if (lane == 0) kept[nk] = kid;
+      ++nk;
+      last = j;
+      if (nk == cap) {
+        cut = j;
+        break;
+      }
+      if (active && !occl && int(lane) > j) {
+        float pd = chain_dist(base + size_t(kid) * dim, own, dim);
+        ++my_pairs;
+        occl = pd < my_d;
+      }
+    }
+    pairs += wave_sum<unsigned long long>(
+        (active && int(lane) <= cut) ? my_pairs : 0u);
+    __syncthreads();
+  }
+  *pair_total += pairs;
+  return nk;
+}
+
+// --------------------------------------------------------- K2: prune (pruned)
+// fusion_prune_reverse_kernel_v0 (Gbuilder.cuh:1105-1244), one wave per node.
+// Candidates: rows[b][0..D) or, in top-1 mode (rows == nullptr), the row the
+// match step would have written: gt[winner[b]][1..gt_k-2] then two tombs
+// (Gbuilder.cuh:121-132).  tomb / self / INT_MAX / out-of-range get FLT_MAX;
+// the list is sorted by (distance, position) which equals the reference's
+// stable bubble sort; the node is skipped if the first entry is a stop id;
+// then strict RNG up to mp kept.  The reverse edges are built afterwards
+// from the pruned lists (reverse_* kernels).
+template <uint32_t DP2>
+__global__ __launch_bounds__(64) void prune_kernel(
+    const float* __restrict__ base, uint32_t n_base, uint32_t dim,
+    const uint32_t* __restrict__ rows, uint32_t D,
+    const uint32_t* __restrict__ gt, const uint32_t* __restrict__ winner,
+    uint32_t gt_k, uint32_t mp, uint32_t* __restrict__ pruned,
+    uint32_t* __restrict__ pcnt, unsigned long long* __restrict__ stats) {
+  __shared__ uint64_t skey[DP2];
+  __shared__ uint32_t sid[DP2];
+  __shared__ uint32_t kept[128];
+  const uint32_t b = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t w = rows ? 0u : winner[b];
+  const float* xb = base + size_t(b) * dim;
+  uint32_t nvalid = 0;
+  for (uint32_t j = lane; j < DP2; j += kWave) {
+    uint32_t id = kTomb;
+    if (j < D) {
+      if (rows) {
+        id = rows[size_t(b) * D + j];
+      } else if (w != kTomb && j + 2 < gt_k) {
+        id = gt[size_t(w) * gt_k + j + 1];
+      }
+    }
+    sid[j] = id;
+    const bool valid =
+        j < D && !(id == kTomb || id == b || id == kMaskedTomb || id >= n_base);
+    float d = FLT_MAX;
+    if (valid) {
+      d = chain_dist(xb, base + size_t(id) * dim, dim);
+      ++nvalid;
+    }
+    skey[j] = j < D ? ((uint64_t(ordered_bits(d)) << 32) | j) : kPadKey;
+  }
+  __syncthreads();
+  bitonic_sort(skey, DP2);
+  unsigned long long pairs = 0;
+  uint32_t nk = 0;
+  const uint32_t id0 = sid[uint32_t(skey[0])];
+  if (!is_stop_id(id0, n_base)) {
+    if (lane == 0) kept[0] = id0;
+    __syncthreads();
+    nk = rng_select<true>(base, n_base, dim, skey, sid, D, mp, kept, 1, &pairs);
+  }
+  for (uint32_t j = lane; j < nk; j += kWave) pruned[size_t(b) * mp + j] = kept[j];
+  if (lane == 0) pcnt[b] = nk;
+  const unsigned long long nd = wave_sum<unsigned long long>(nvalid);
+  if (lane == 0) {
+    stat_add(stats, ST_PRUNE_DIST, nd);
+    stat_add(stats, ST_PRUNE_PAIRS, pairs);
+  }
+}
+
+// ------------------------------------------------------ reverse edges (CSR)
+// add_reverse (Gbuilder.cuh:1020-1038) keeps whichever sources win the
+// atomicAdd race; here every edge b -> c is recorded in a CSR keyed by c and
+// the merge step keeps the cap smallest sources (rule C2).
+__global__ void reverse_count_kernel(const uint32_t* __restrict__ pruned,
+                                     const uint32_t* __restrict__ pcnt,
+                                     uint32_t n_base, uint32_t mp,
+                                     uint32_t* __restrict__ deg) {
+  const uint64_t total = uint64_t(n_base) * mp;
+  for (uint64_t e = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; e < total;
+       e += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t b = uint32_t(e / mp), j = uint32_t(e % mp);
+    if (j < pcnt[b]) {
+      const uint32_t c = pruned[e];
+      if (c < n_base) atomicAdd(&deg[c], 1u);
+    }
+  }
+}
+
+__global__ void reverse_fill_kernel(const uint32_t* __restrict__ pruned,
+                                    const uint32_t* __restrict__ pcnt,
+                                    uint32_t n_base, uint32_t mp,
+                                    const uint32_t* __restrict__ off,
+                                    uint32_t* __restrict__ cursor,
+                                    uint32_t* __restrict__ srcs) {
+  const uint64_t total = uint64_t(n_base) * mp;
+  for (uint64_t e = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; e < total;
+       e += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t b = uint32_t(e / mp), j = uint32_t(e % mp);
+    if (j < pcnt[b]) {
+      const uint32_t c = pruned[e];
+      if (c < n_base) {
+        const uint32_t slot = atomicAdd(&cursor[c], 1u);
+        srcs[off[c] + slot] = b;
+      }
+    }
+  }
+}
+
+// Exclusive scan, 3 passes: per-block scan of 1024 entries, scan of the
+// block sums (one block), add-back.  in has n entries, out n+1.
+constexpr uint32_t kScanBlock = 1024;
+
+__global__ __launch_bounds__(256) void scan_blocks_kernel(
+    const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
+    uint32_t* __restrict__ block_sums) {
+  __shared__ uint32_t wsum[4];
+  const uint32_t base_i = blockIdx.x * kScanBlock;
+  const uint32_t t = threadIdx.x;
+  uint32_t v[4];
+  uint32_t run = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = base_i + t * 4 + k;
+    v[k] = i < n ? in[i] : 0u;
+    run += v[k];
+  }
+  // inclusive scan of per-thread totals across the block
+  uint32_t x = run;
+  const uint32_t lane = t & 63u, wv = t >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= uint32_t(o)) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  uint32_t wprefix = 0;
+  for (uint32_t k = 0; k < wv; ++k) wprefix += wsum[k];
+  uint32_t excl = wprefix + x - run;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = base_i + t * 4 + k;
+    if (i < n) out[i] = excl;
+    excl += v[k];
+  }
+  if (t == 255) block_sums[blockIdx.x] = wprefix + x;
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(
+    uint32_t* __restrict__ sums, uint32_t nb, uint32_t* __restrict__ total) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base_i = 0; base_i < nb; base_i += 1024) {
+    const uint32_t i = base_i + t;
+    const uint32_t v = i < nb ? sums[i] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= uint32_t(o)) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    uint32_t wprefix = 0;
+    for (uint32_t k = 0; k < wv; ++k) wprefix += wsum[k];
+    const uint32_t c = carry;
+    if (i < nb) sums[i] = c + wprefix + x - v;
+    __syncthreads();
+    if (t == 1023) carry = c + wprefix + x;
+    __syncthreads();
+  }
+  if (t == 0) *total = carry;
+}
+
+__global__ void scan_add_kernel(uint32_t* __restrict__ out, uint32_t n,
+                                const uint32_t* __restrict__ block_sums) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += block_sums[i / kScanBlock];
+}
+
+// ------------------------------------------------------------ K3: merge
+// merge_and_sort + fusion_merge_sort_prune_kernel (Gbuilder.cuh:1433-1601),
+// one wave per node c.  Candidates: the cap smallest reverse sources of c
+// in ascending id (C2), then c's pruned list.  Stable distance sort, list[0]
+// kept unconditionally, strict RNG to `deg`, non-strict fill when !strict
+// (Gbuilder.cuh:1559-1594), tomb fill.
+constexpr uint32_t kSegChunk = 512;
+
+__global__ __launch_bounds__(64) void merge_kernel(
+    const float* __restrict__ base, uint32_t n_base, uint32_t dim,
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ srcs,
+    uint32_t cap, const uint32_t* __restrict__ pruned,
+    const uint32_t* __restrict__ pcnt, uint32_t mp, uint32_t deg,
+    int strict, uint32_t* __restrict__ out,
+    unsigned long long* __restrict__ stats) {
+  __shared__ uint32_t buf[2 * kSegChunk];
+  __shared__ uint64_t skey[128];
+  __shared__ uint32_t sid[128];
+  __shared__ uint32_t kept[128];
+  __shared__ uint8_t added[128];
+  const uint32_t c = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t s0 = off[c], seg = off[c + 1] - s0;
+  const uint32_t r = seg < cap ? seg : cap;
+  // ---- select the r smallest sources, ascending
+  if (seg <= kSegChunk) {
+    uint32_t n2 = 64;
+    while (n2 < seg) n2 <<= 1;
+    for (uint32_t j = lane; j < n2; j += kWave) buf[j] = j < seg ? srcs[s0 + j] : kTomb;
+    __syncthreads();
+    if (seg > 1) bitonic_sort(buf, n2);
+  } else {
+    for (uint32_t j = lane; j < kSegChunk; j += kWave) buf[j] = srcs[s0 + j];
+    __syncthreads();
+    bitonic_sort(buf, kSegChunk);
+    for (uint32_t done = kSegChunk; done < seg; done += kSegChunk) {
+      for (uint32_t j = lane; j < kSegChunk; j += kWave) {
+        if (j >= cap) buf[j] = kTomb;
+        const uint32_t g = done + j;
+        buf[kSegChunk + j] = g < seg ? srcs[s0 + g] : kTomb;
+      }
+      __syncthreads();
+      bitonic_sort(buf, 2 * kSegChunk);
+    }
+  }
+  // ---- candidate list: reverse part then pruned part
+  const uint32_t p = pcnt[c];
+  const uint32_t n = r + p;
+  const float* xc = base + size_t(c) * dim;
+  for (uint32_t j = lane; j < 128; j += kWave) {
+    uint32_t id = kTomb;
+    if (j < r) id = buf[j];
+    else if (j < n) id = pruned[size_t(c) * mp + (j - r)];
+    sid[j] = id;
+    float d = FLT_MAX;
+    if (j < n) d = chain_dist(xc, base + size_t(id) * dim, dim);
+    skey[j] = j < n ? ((uint64_t(ordered_bits(d)) << 32) | j) : kPadKey;
+  }
+  __syncthreads();
+  uint32_t* o = out + size_t(c) * deg;
+  if (n == 0) {
+    for (uint32_t j = lane; j < deg; j += kWave) o[j] = kTomb;
+    return;
+  }
+  bitonic_sort(skey, 128u);
+  unsigned long long pairs = 0;
+  if (lane == 0) kept[0] = sid[uint32_t(skey[0])];
+  __syncthreads();
+  uint32_t nk =
+      rng_select<false>(base, n_base, dim, skey, sid, n, deg, kept, 1, &pairs);
+  if (!strict && nk < deg) {
+    // mark the first list slot equal to each kept id, then append the
+    // unmarked entries in sorted order (duplicates can be appended again).
+    for (uint32_t j = lane; j < 128; j += kWave) added[j] = 0;
+    __syncthreads();
+    for (uint32_t a = 0; a < nk; ++a) {
+      const uint32_t kid = kept[a];
+      uint32_t first = 0xFFFFFFFFu;
+      for (uint32_t j0 = 0; j0 < n && first == 0xFFFFFFFFu; j0 += kWave) {
+        const uint32_t j = j0 + lane;
+        const uint64_t m = __ballot(j < n && sid[uint32_t(skey[j])] == kid);
+        if (m) first = j0 + __ffsll((long long)m) - 1;
+      }
+      if (lane == 0 && first != 0xFFFFFFFFu) added[first] = 1;
+      __syncthreads();
+    }
+    if (lane == 0) {
+      for (uint32_t i = 0; i < n && nk < deg; ++i) {
+        if (added[i]) continue;
+        const uint32_t id = sid[uint32_t(skey[i])];
+        if (id != kTomb && id != c && id < n_base) kept[nk++] = id;
+      }
+    }
+    nk = __shfl(nk, 0, 64);
+    __syncthreads();
+  }
+  for (uint32_t j = lane; j < deg; j += kWave) o[j] = j < nk ? kept[j] : kTomb;
+  if (lane == 0) {
+    stat_add(stats, ST_MERGE_DIST, n);
+    stat_add(stats, ST_MERGE_PAIRS, pairs);
+    if (seg > cap) stat_add(stats, ST_REV_OVERFLOW, 1);
+  }
+}
+
+// ------------------------------------------------------------- K4: search
+// link_process_v0_store_base_data (Gsearcher.cuh:896-1066), one wave per
+// query.  LDS per wave: the visited set (open addressing, exact), the list
+// as u64 keys (distance, id, expanded bit), the candidate slots, the
+// compacted miss list and the Kp expanded ids.  Distances: half a wave (32
+// lanes) per candidate, lane l accumulating elements l, l+32, ... with the
+// query slice held in registers, then the xor tree 16/8/4/2/1, whose lane-0
+// result equals the reference's __shfl_down_sync tree.
+__device__ __forceinline__ uint32_t visit_hash(uint32_t id, uint32_t mask) {
+  return (id * 2654435761u) & mask;
+}
+
+__device__ __forceinline__ bool visit_insert(uint32_t* tab, uint32_t mask,
+                                             uint32_t id) {
+  const uint32_t key = id + 1u;
+  uint32_t s = visit_hash(id, mask);
+  while (true) {
+    const uint32_t old = atomicCAS(&tab[s], 0u, key);
+    if (old == 0u) return true;
+    if (old == key) return false;
+    s = (s + 1u) & mask;
+  }
+}
+
+template <int TQ, int U>
+__device__ __forceinline__ void eval_misses(
+    const float (&qv)[TQ], const float* __restrict__ base, uint32_t dim,
+    const uint32_t* cand, const uint32_t* todo, uint32_t ntodo, uint64_t* tail) {
+  const uint32_t lane = lane_id();
+  const uint32_t half = lane >> 5, l = lane & 31u;
+  for (uint32_t p0 = 0; p0 < ntodo; p0 += 2 * U) {
+    uint32_t tt[U], cid[U];
+    bool ok[U];
+    float xv[U][TQ];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t idx = p0 + 2 * u + half;
+      ok[u] = idx < ntodo;
+      tt[u] = ok[u] ? todo[idx] : 0u;
+      cid[u] = ok[u] ? cand[tt[u]] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float* row = base + size_t(cid[u]) * dim;
+#pragma unroll
+      for (int t = 0; t < TQ; ++t) {
+        const uint32_t e = l + 32u * t;
+        xv[u][t] = (ok[u] && e < dim) ? row[e] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      float s = 0.0f;
+#pragma unroll
+      for (int t = 0; t < TQ; ++t) {
+        if (l + 32u * t < dim) s = __builtin_fmaf(qv[t], xv[u][t], s);
+      }
+#pragma unroll
+      for (int o = 16; o > 0; o >>= 1) s = s + __shfl_xor(s, o, 64);
+      if (l == 0 && ok[u]) tail[tt[u]] = search_key(-s, cid[u]);
+    }
+  }
+}
+
+template <int TQ>
+__global__ __launch_bounds__(64) void search_kernel(
+    const float* __restrict__ queries, uint32_t n_query,
+    const float* __restrict__ base, uint32_t n_base, uint32_t dim,
+    const uint32_t* __restrict__ graph, uint32_t gdeg, uint32_t Km,
+    uint32_t Kp, uint32_t Kd, uint32_t topk, uint32_t reset_iter,
+    uint32_t entry_seed, uint32_t max_iters, uint32_t tsize, uint32_t lp2,
+    uint32_t* __restrict__ out, unsigned long long* __restrict__ stats) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  const uint32_t kpkd = Kp * Kd;
+  uint32_t* tab = reinterpret_cast<uint32_t*>(smem);
+  uint64_t* keys = reinterpret_cast<uint64_t*>(smem + size_t(tsize) * 4);
+  uint32_t* cand = reinterpret_cast<uint32_t*>(keys + lp2);
+  uint32_t* todo = cand + kpkd;
+  uint32_t* explore = todo + kpkd;
+  const uint32_t q = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t l = lane & 31u;
+  const uint32_t tmask = tsize - 1u;
+  float qv[TQ];
+#pragma unroll
+  for (int t = 0; t < TQ; ++t) {
+    const uint32_t e = l + 32u * t;
+    qv[t] = e < dim ? queries[size_t(q) * dim + e] : 0.0f;
+  }
+  // initial list: [0, Km) = (FLT_MAX, tomb); [Km, Km+kpkd) = entry points
+  for (uint32_t i = lane; i < lp2; i += kWave)
+    keys[i] = i < Km ? search_key(FLT_MAX, kTomb) : kPadKey;
+  for (uint32_t t = lane; t < kpkd; t += kWave) {
+    cand[t] = uint32_t((uint64_t(entry_seed) + t) % n_base);
+    todo[t] = t;
+  }
+  __syncthreads();
+  uint64_t* tail = keys + Km;
+  eval_misses<TQ, 4>(qv, base, dim, cand, todo, kpkd, tail);
+  unsigned long long ndist = kpkd;
+  __syncthreads();
+  bitonic_sort(keys, lp2);
+  auto reset_table = [&]() {
+    for (uint32_t i = lane; i < tsize; i += kWave) tab[i] = 0u;
+    __syncthreads();
+    for (uint32_t i = lane; i < Km; i += kWave)
+      visit_insert(tab, tmask, key_raw_id(keys[i]) & kMaskedTomb);
+    __syncthreads();
+  };
+  reset_table();
+  uint32_t iter = 0, total_iter = 0;
+  bool capped = false;
+  while (true) {
+    // collect_top_p_normal (Gsearcher.cuh:75-106): first Kp unexpanded
+    uint32_t found = 0;
+    for (uint32_t c0 = 0; c0 < Km && found < Kp; c0 += kWave) {
+      const uint32_t i = c0 + lane;
+      uint64_t k = 0;
+      bool un = false;
+      if (i < Km) {
+        k = keys[i];
+        un = (k & 1ull) == 0;
+      }
+      const uint64_t m = __ballot(un);
+      const uint32_t rank = __popcll(m & lanemask_lt());
+      if (un && found + rank < Kp) {
+        explore[found + rank] = key_raw_id(k);
+        keys[i] = k | 1ull;
+      }
+      const uint32_t pc = __popcll(m);
+      found += pc < Kp - found ? pc : Kp - found;
+    }
+    if (found == 0) break;
+    if (total_iter >= max_iters) {
+      capped = true;
+      break;
+    }
+    for (uint32_t e = found + lane; e < Kp; e += kWave) explore[e] = kTomb;
+    __syncthreads();
+    // neighbour ids of the expanded nodes (first Kd of each graph row)
+    for (uint32_t t = lane; t < kpkd; t += kWave) {
+      const uint32_t e = t / Kd, j = t - e * Kd;
+      const uint32_t eid = explore[e];
+      cand[t] = eid == kTomb ? kTomb : graph[size_t(eid) * gdeg + j];
+    }
+    __syncthreads();
+    // visited test-and-set; misses go to the todo list, hits get FLT_MAX
+    uint32_t ntodo = 0;
+    for (uint32_t c0 = 0; c0 < kpkd; c0 += kWave) {
+      const uint32_t t = c0 + lane;
+      bool miss = false;
+      if (t < kpkd) {
+        const uint32_t id = cand[t];
+        if (id != kTomb) miss = visit_insert(tab, tmask, id);
+        if (!miss) tail[t] = search_key(FLT_MAX, id);
+      }
+      const uint64_t m = __ballot(miss);
+      if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = t;
+      ntodo += __popcll(m);
+    }
+    __syncthreads();
+    eval_misses<TQ, 4>(qv, base, dim, cand, todo, ntodo, tail);
+    ndist += ntodo;
+    __syncthreads();
+    bitonic_sort(keys, lp2);
+    ++total_iter;
+    if (++iter == reset_iter) {
+      reset_table();
+      iter = 0;
+    }
+  }
+  for (uint32_t i = lane; i < topk; i += kWave)
+    out[size_t(q) * topk + i] = key_raw_id(keys[i]) & kMaskedTomb;
+  if (lane == 0) {
+    stat_add(stats, ST_SEARCH_DIST, ndist);
+    stat_add(stats, ST_SEARCH_ITERS, total_iter);
+    if (capped) stat_add(stats, ST_SEARCH_CAPPED, 1);
+  }
+}
+
+// ------------------------------------------------------------ launchers
+
+static inline uint32_t grid_for(uint64_t n, uint32_t block) {
+  uint64_t g = (n + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g == 0) g = 1;
+  return uint32_t(g);
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v,
+                           hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_u32_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n,
+                           unsigned long long v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  fill_u64_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_match(const uint32_t* gt, uint32_t n_query, uint32_t gt_k,
+                        uint32_t n_base, uint32_t* winner, hipStream_t s) {
+  hipError_t e = launch_fill_u32(winner, n_base, kTomb, s);
+  if (e != hipSuccess) return e;
+  if (n_query == 0) return hipSuccess;
+  match_winner_kernel<<<grid_for(n_query, 256), 256, 0, s>>>(gt, n_query, gt_k,
+                                                             n_base, winner);
+  return hipGetLastError();
+}
+
+hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
+                        const uint32_t* rows, uint32_t D, const uint32_t* gt,
+                        const uint32_t* winner, uint32_t gt_k, uint32_t mp,
+                        uint32_t* pruned, uint32_t* pcnt,
+                        unsigned long long* stats, hipStream_t s) {
+  if (n_base == 0) return hipSuccess;
+#define PG_PRUNE(DP)                                                     \
+  prune_kernel<DP><<<n_base, 64, 0, s>>>(base, n_base, dim, rows, D, gt, \
+                                         winner, gt_k, mp, pruned, pcnt, \
+                                         stats)
+  if (D <= 64) PG_PRUNE(64);
+  else if (D <= 128) PG_PRUNE(128);
+  else if (D <= 256) PG_PRUNE(256);
+  else if (D <= 512) PG_PRUNE(512);
+  else return hipErrorInvalidValue;
+#undef PG_PRUNE
+  return hipGetLastError();
+}
+
+hipError_t launch_reverse(const uint32_t* pruned, const uint32_t* pcnt,
+                          uint32_t n_base, uint32_t mp, uint32_t* deg,
+                          uint32_t* off, uint32_t* cursor, uint32_t* scan_tmp,
+                          uint32_t* srcs, hipStream_t s) {
+  hipError_t e = launch_fill_u32(deg, n_base, 0u, s);
+  if (e != hipSuccess) return e;
+  e = launch_fill_u32(cursor, n_base, 0u, s);
+  if (e != hipSuccess) return e;
+  const uint64_t total = uint64_t(n_base) * mp;
+  if (total == 0) return launch_fill_u32(off, uint64_t(n_base) + 1, 0u, s);
+  reverse_count_kernel<<<grid_for(total, 256), 256, 0, s>>>(pruned, pcnt,
+                                                            n_base, mp, deg);
+  const uint32_t nb = (n_base + kScanBlock - 1) / kScanBlock;
+  scan_blocks_kernel<<<nb, 256, 0, s>>>(deg, n_base, off, scan_tmp);
+  scan_sums_kernel<<<1, 1024, 0, s>>>(scan_tmp, nb, off + n_base);
+  scan_add_kernel<<<(n_base + 255) / 256, 256, 0, s>>>(off, n_base, scan_tmp);
+  reverse_fill_kernel<<<grid_for(total, 256), 256, 0, s>>>(
+      pruned, pcnt, n_base, mp, off, cursor, srcs);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
+                        const uint32_t* off, const uint32_t* srcs,
+                        uint32_t cap, const uint32_t* pruned,
+                        const uint32_t* pcnt, uint32_t mp, uint32_t deg,
+                        int strict, uint32_t* out, unsigned long long* stats,
+                        hipStream_t s) {
+  if (n_base == 0) return hipSuccess;
+  merge_kernel<<<n_base, 64, 0, s>>>(base, n_base, dim, off, srcs, cap,
+                                     pruned, pcnt, mp, deg, strict, out, stats);
+  return hipGetLastError();
+}
+
+size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
+                         uint32_t Kp) {
+  return size_t(table_size) * 4 + size_t(lp2) * 8 + size_t(kpkd) * 8 +
+         size_t(Kp) * 4 + 16;
+}
+
+hipError_t launch_search(const float* queries, uint32_t n_query,
+                         const float* base, uint32_t n_base, uint32_t dim,
+                         const uint32_t* graph, uint32_t gdeg, uint32_t Km,
+                         uint32_t Kp, uint32_t Kd, uint32_t topk,
+                         uint32_t reset_iter, uint32_t entry_seed,
+                         uint32_t max_iters, uint32_t table_size,
+                         uint32_t* out, unsigned long long* stats,
+                         hipStream_t s) {
+  if (n_query == 0) return hipSuccess;
+  uint32_t lp2 = 64;
+  while (lp2 < Km + Kp * Kd) lp2 <<= 1;
+  const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp);
+  const uint32_t tq = (dim + 31) / 32;
+#define PG_SEARCH(T)                                                          \
+  do {                                                                        \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&search_kernel<T>), \
+                        hipFuncAttributeMaxDynamicSharedMemorySize,           \
+                        int(smem));                                           \
+    search_kernel<T><<<n_query, 64, smem, s>>>(                               \
+        queries, n_query, base, n_base, dim, graph, gdeg, Km, Kp, Kd, topk,   \
+        reset_iter, entry_seed, max_iters, table_size, lp2, out, stats);      \
+  } while (0)
+  if (tq <= 4) PG_SEARCH(4);
+  else if (tq <= 8) PG_SEARCH(8);
+  else if (tq <= 16) PG_SEARCH(16);
+  else if (tq <= 32) PG_SEARCH(32);
+  else return hipErrorInvalidValue;
+#undef PG_SEARCH
+  return hipGetLastError();
+}
+
+}  // namespace pg

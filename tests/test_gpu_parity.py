@@ -1,0 +1,197 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle, bit-exact
+on neighbour ids, at sizes the oracle finishes in seconds.
+
+Calls go through the C ABI (libparagraph_hip.so via ctypes); inputs are
+seeded synthetic workloads of the benchmark shapes, scaled down in N.
+"""
+import numpy as np
+import pytest
+
+from oracle import cpu_ref
+from paragraph_amd import _native, datagen
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # name: (n_base, dim, n_comp, M, L, normalize, zipf, gt_len_range)
+    "d200_m32_l100": (2000, 200, 64, 32, 100, False, None, None),
+    "d512_m64_l160_norm": (1500, 512, 256, 64, 160, True, None, None),
+    "d200_m48_l128_zipf_ragged": (1500, 200, 1024, 48, 128, True, 1.1,
+                                  (16, 128)),
+    "d64_m32_l64": (1200, 64, 16, 32, 64, False, None, None),
+}
+
+_cache = {}
+
+
+def workload(name):
+    if name not in _cache:
+        n, dim, nc, M, L, norm, zipf, gl = CASES[name]
+        w = datagen.make_workload(n, dim, nc, normalize=norm, zipf=zipf,
+                                  gt_len_range=gl, device="cpu")
+        w["gt_u32"] = np.ascontiguousarray(w["gt"].view(np.uint32))
+        w["params"] = _native.make_params(n, dim, n, M, L)
+        _cache[name] = w
+    return _cache[name]
+
+
+def to_dev(a, torch):
+    if a.dtype == np.uint32:
+        return torch.from_numpy(a.view(np.int32)).cuda()
+    return torch.from_numpy(a).cuda()
+
+
+def from_dev(t):
+    a = t.cpu().numpy()
+    return a.view(np.uint32) if a.dtype == np.int32 else a
+
+
+def orc_params(p):
+    d = p.as_dict()
+    return {k: d[k] for k in (
+        "n_base", "dim", "n_query", "gt_k", "proj_deg", "proj_pr", "proj_rev",
+        "M", "link_rev", "Km", "Kp", "Kd", "topk", "reset_iter", "rounds",
+        "entry_seed", "max_iters")}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_stages_match_oracle(name, cuda):
+    import torch
+    w = workload(name)
+    p = w["params"]
+    n, dim = w["base"].shape
+    print(f"[parity] stages {name} n={n} dim={dim}", flush=True)
+    base_d = to_dev(w["base"], torch)
+    gt_d = to_dev(w["gt_u32"], torch)
+    i32 = dict(dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        # K0 match
+        winner_d = torch.empty(n, **i32)
+        ctx.run_match(gt_d, winner_d)
+        winner = cpu_ref.match_winner(w["gt_u32"], n)
+        ctx.sync()
+        np.testing.assert_array_equal(from_dev(winner_d), winner)
+        # K2 on top-1 rows
+        pr_d = torch.full((n, p.proj_pr), -1, **i32)
+        pc_d = torch.zeros(n, **i32)
+        ctx.run_prune_top1(base_d, gt_d, winner_d, p.proj_pr, pr_d, pc_d)
+        rows = cpu_ref.top1_rows(w["gt_u32"], winner)
+        pr, pc = cpu_ref.prune(w["base"], rows, p.proj_pr)
+        ctx.sync()
+        np.testing.assert_array_equal(from_dev(pc_d), pc)
+        mask = np.arange(p.proj_pr)[None, :] < pc[:, None]
+        np.testing.assert_array_equal(from_dev(pr_d)[mask], pr[mask])
+        # reverse CSR: same offsets, same multiset per segment
+        off_d = torch.zeros(n + 1, **i32)
+        srcs_d = torch.zeros(max(1, int(pc.sum())), **i32)
+        ctx.run_reverse(pr_d, pc_d, off_d, srcs_d)
+        off, srcs = cpu_ref.reverse_csr(pr, pc)
+        ctx.sync()
+        off_g = from_dev(off_d)
+        np.testing.assert_array_equal(off_g, off)
+        srcs_g = from_dev(srcs_d)[: off[-1]]
+        for c in range(n):
+            a, b = off[c], off[c + 1]
+            if b - a > 1:
+                srcs_g[a:b] = np.sort(srcs_g[a:b])
+        np.testing.assert_array_equal(srcs_g, srcs)
+        # K3 projection (non-strict)
+        proj_d = torch.empty((n, p.proj_deg), **i32)
+        ctx.run_merge(base_d, off_d, srcs_d, p.proj_rev, pr_d, pc_d,
+                      p.proj_deg, False, proj_d)
+        proj = cpu_ref.merge(w["base"], off, srcs, p.proj_rev, pr, pc,
+                             p.proj_deg, False)
+        ctx.sync()
+        np.testing.assert_array_equal(from_dev(proj_d), proj)
+        # K4 search over the projection graph
+        res_d = torch.empty((n, p.topk), **i32)
+        ctx.run_search(base_d, base_d, proj_d, res_d)
+        res = cpu_ref.search(w["base"], w["base"], proj, p.Km, p.Kp, p.Kd,
+                             p.topk)
+        ctx.sync()
+        res_g = from_dev(res_d)
+        bad = np.nonzero((res_g != res).any(axis=1))[0]
+        assert bad.size == 0, f"search rows differ: {bad[:10]} of {n}"
+        # K2 on search rows, reverse, K3 strict
+        pr2_d = torch.full((n, p.M), -1, **i32)
+        pc2_d = torch.zeros(n, **i32)
+        ctx.run_prune_rows(base_d, res_d, p.M, pr2_d, pc2_d)
+        pr2, pc2 = cpu_ref.prune(w["base"], res, p.M)
+        ctx.sync()
+        np.testing.assert_array_equal(from_dev(pc2_d), pc2)
+        mask = np.arange(p.M)[None, :] < pc2[:, None]
+        np.testing.assert_array_equal(from_dev(pr2_d)[mask], pr2[mask])
+        off2, srcs2 = cpu_ref.reverse_csr(pr2, pc2)
+        srcs2_d = torch.zeros(max(1, int(pc2.sum())), **i32)
+        ctx.run_reverse(pr2_d, pc2_d, off_d, srcs2_d)
+        link_d = torch.empty((n, p.M), **i32)
+        ctx.run_merge(base_d, off_d, srcs2_d, p.link_rev, pr2_d, pc2_d, p.M,
+                      True, link_d)
+        link = cpu_ref.merge(w["base"], off2, srcs2, p.link_rev, pr2, pc2,
+                             p.M, True)
+        ctx.sync()
+        np.testing.assert_array_equal(from_dev(link_d), link)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_whole_stage_matches_oracle(name, cuda):
+    import torch
+    w = workload(name)
+    p = w["params"]
+    n = p.n_base
+    print(f"[parity] whole stage {name}", flush=True)
+    st_o = cpu_ref.OrcStats()
+    proj_o, link_o = cpu_ref.gpu_stage(orc_params(p), w["base"], w["gt_u32"],
+                                       stats=st_o)
+    base_d = to_dev(w["base"], torch)
+    gt_d = to_dev(w["gt_u32"], torch)
+    proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=cuda)
+    link_d = torch.empty((n, p.M), dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.stage_device(base_d, gt_d, proj_d, link_d)
+        ctx.wait()
+        st = ctx.stats().as_dict()
+    np.testing.assert_array_equal(from_dev(proj_d), proj_o)
+    np.testing.assert_array_equal(from_dev(link_d), link_o)
+    so = st_o.as_dict()
+    for k in ("prune_dist", "prune_pairs", "merge_dist", "merge_pairs",
+              "search_dist", "search_iters", "search_capped", "rev_overflow"):
+        assert st[k] == so[k], (k, st[k], so[k])
+
+
+def test_host_entry_point_matches_device_entry_point(cuda):
+    import torch
+    w = workload("d64_m32_l64")
+    p = w["params"]
+    n = p.n_base
+    proj_h = np.empty((n, p.proj_deg), np.uint32)
+    link_h = np.empty((n, p.M), np.uint32)
+    with _native.Context(p) as ctx:
+        ctx.stage_host(w["base"], w["gt_u32"], proj_h, link_h)
+        ctx.wait()
+        proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=cuda)
+        link_d = torch.empty((n, p.M), dtype=torch.int32, device=cuda)
+        ctx.stage_device(to_dev(w["base"], torch), to_dev(w["gt_u32"], torch),
+                         proj_d, link_d)
+        ctx.wait()
+    np.testing.assert_array_equal(proj_h, from_dev(proj_d))
+    np.testing.assert_array_equal(link_h, from_dev(link_d))
+
+
+def test_build_is_deterministic(cuda):
+    import torch
+    w = workload("d200_m32_l100")
+    p = w["params"]
+    n = p.n_base
+    outs = []
+    with _native.Context(p) as ctx:
+        for _ in range(2):
+            proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32,
+                                 device=cuda)
+            link_d = torch.empty((n, p.M), dtype=torch.int32, device=cuda)
+            ctx.stage_device(to_dev(w["base"], torch),
+                             to_dev(w["gt_u32"], torch), proj_d, link_d)
+            ctx.wait()
+            outs.append((from_dev(proj_d), from_dev(link_d)))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
