@@ -124,6 +124,11 @@ int pg_wait(pg_ctx* ctx);   /* blocks until the last enqueued work ends */
 int pg_query(pg_ctx* ctx);  /* PG_OK if done, PG_ERR_BUSY if in flight */
 int pg_get_stats(pg_ctx* ctx, pg_stats* out);  /* waits, then copies */
 int pg_reset_stats(pg_ctx* ctx);
+/* Counters accumulated during pipeline stage `stage` (0 .. n_stages-1) of
+ * the last build, in pg_stats field order (prune_dist, prune_pairs,
+ * merge_dist, merge_pairs, search_dist, search_iters, search_capped,
+ * rev_overflow, screen_fallback, 7 reserved). */
+int pg_get_stage_counters(pg_ctx* ctx, uint32_t stage, uint64_t* out16);
 const char* pg_last_error(const pg_ctx* ctx);
 const char* pg_stage_name(uint32_t i);
 

@@ -453,6 +453,84 @@ int orc_gpu_stage(const orc_params* p, const float* base, const uint32_t* gt,
   return 0;
 }
 
+// Node-subset variants for timing on a bounded sample: the same per-node
+// work as orc_prune / orc_merge, for nodes[0..n_nodes) only.  rows_s holds
+// the sample's candidate rows (n_nodes x D); outputs are per sample entry.
+void orc_prune_nodes(const float* base, uint32_t n_base, uint32_t dim,
+                     const uint32_t* nodes, uint32_t n_nodes,
+                     const uint32_t* rows_s, uint32_t D, uint32_t mp,
+                     uint32_t* pruned_s, uint32_t* pcnt_s) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t si = 0; si < int64_t(n_nodes); ++si) {
+    const uint32_t b = nodes[si];
+    const uint32_t* row = rows_s + size_t(si) * D;
+    const float* xb = base + size_t(b) * dim;
+    std::vector<Cand> c(D);
+    for (uint32_t j = 0; j < D; ++j) {
+      uint32_t id = row[j];
+      c[j].id = id;
+      c[j].d = (id == kTomb || id == b || id == kMaskedTomb || id >= n_base)
+                   ? FLT_MAX
+                   : chain_dist(xb, base + size_t(id) * dim, dim);
+    }
+    stable_by_dist(c);
+    pcnt_s[si] = 0;
+    if (is_stop_id(c[0].id, n_base)) continue;
+    std::vector<uint32_t> kept{c[0].id};
+    for (uint32_t i = 1; i < D && kept.size() < mp; ++i) {
+      if (is_stop_id(c[i].id, n_base)) break;
+      bool occl;
+      occluded_by(base, dim, c[i], kept, &occl);
+      if (!occl) kept.push_back(c[i].id);
+    }
+    for (size_t k = 0; k < kept.size(); ++k) pruned_s[size_t(si) * mp + k] = kept[k];
+    pcnt_s[si] = uint32_t(kept.size());
+  }
+}
+
+void orc_merge_nodes(const float* base, uint32_t n_base, uint32_t dim,
+                     const uint32_t* nodes, uint32_t n_nodes,
+                     const uint32_t* off, const uint32_t* srcs, uint32_t cap,
+                     const uint32_t* pruned, const uint32_t* pcnt, uint32_t mp,
+                     uint32_t deg, int strict, uint32_t* out_s) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t si = 0; si < int64_t(n_nodes); ++si) {
+    const uint32_t c = nodes[si];
+    uint32_t* o = out_s + size_t(si) * deg;
+    const uint32_t seg = off[c + 1] - off[c];
+    const uint32_t r = std::min(seg, cap), p = pcnt[c], n = r + p;
+    for (uint32_t k = 0; k < deg; ++k) o[k] = kTomb;
+    if (n == 0) continue;
+    const float* xc = base + size_t(c) * dim;
+    std::vector<Cand> l(n);
+    for (uint32_t j = 0; j < r; ++j) l[j].id = srcs[off[c] + j];
+    for (uint32_t j = 0; j < p; ++j) l[r + j].id = pruned[size_t(c) * mp + j];
+    for (uint32_t j = 0; j < n; ++j)
+      l[j].d = chain_dist(xc, base + size_t(l[j].id) * dim, dim);
+    stable_by_dist(l);
+    std::vector<uint32_t> kept{l[0].id};
+    for (uint32_t i = 1; i < n && kept.size() < deg; ++i) {
+      bool occl;
+      occluded_by(base, dim, l[i], kept, &occl);
+      if (!occl) kept.push_back(l[i].id);
+    }
+    if (!strict && kept.size() < deg) {
+      std::vector<char> added(n, 0);
+      const size_t k = kept.size();
+      for (size_t a = 0; a < k; ++a)
+        for (uint32_t j = 0; j < n; ++j)
+          if (l[j].id == kept[a]) {
+            added[j] = 1;
+            break;
+          }
+      for (uint32_t i = 0; i < n && kept.size() < deg; ++i)
+        if (!added[i] && l[i].id != kTomb && l[i].id != c && l[i].id < n_base)
+          kept.push_back(l[i].id);
+    }
+    for (size_t a = 0; a < kept.size(); ++a) o[a] = kept[a];
+  }
+}
+
 int orc_max_threads() { return omp_get_max_threads(); }
 void orc_set_threads(int n) { omp_set_num_threads(n); }
 

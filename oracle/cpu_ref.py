@@ -67,6 +67,10 @@ def _declare(L):
                             ctypes.c_int, _P, _P]
     L.orc_search.argtypes = [_P, _U64, _P, _U32, _U32, _P, _U32, _U32, _U32,
                              _U32, _U32, _U32, _U32, _U32, _P, _P]
+    L.orc_prune_nodes.argtypes = [_P, _U32, _U32, _P, _U32, _P, _U32, _U32,
+                                  _P, _P]
+    L.orc_merge_nodes.argtypes = [_P, _U32, _U32, _P, _U32, _P, _P, _U32, _P,
+                                  _P, _U32, _U32, ctypes.c_int, _P]
     L.orc_gpu_stage.argtypes = [ctypes.POINTER(OrcParams), _P, _P, _P, _P, _P]
     L.orc_gpu_stage.restype = ctypes.c_int
     L.orc_max_threads.restype = ctypes.c_int
@@ -161,6 +165,31 @@ def search(queries, base, graph, Km, Kp, Kd, topk, reset_iter=15,
                      graph.shape[1], Km, Kp, Kd, topk, reset_iter, entry_seed,
                      max_iters, _ptr(out),
                      ctypes.byref(stats) if stats is not None else None)
+    return out
+
+
+def prune_nodes(base, nodes, rows_s, mp):
+    """orc_prune restricted to `nodes` (rows_s: their candidate rows)."""
+    base, nodes, rows_s = _f32(base), _u32(nodes), _u32(rows_s)
+    n, dim = base.shape
+    pruned = np.full((nodes.shape[0], mp), TOMB, np.uint32)
+    pcnt = np.zeros(nodes.shape[0], np.uint32)
+    lib().orc_prune_nodes(_ptr(base), n, dim, _ptr(nodes), nodes.shape[0],
+                          _ptr(rows_s), rows_s.shape[1], mp, _ptr(pruned),
+                          _ptr(pcnt))
+    return pruned, pcnt
+
+
+def merge_nodes(base, nodes, off, srcs, cap, pruned, pcnt, deg, strict):
+    """orc_merge restricted to `nodes` (CSR and pruned lists cover all N)."""
+    base, nodes, off = _f32(base), _u32(nodes), _u32(off)
+    pruned, pcnt = _u32(pruned), _u32(pcnt)
+    srcs = _u32(srcs) if len(srcs) else np.zeros(1, np.uint32)
+    n, dim = base.shape
+    out = np.empty((nodes.shape[0], deg), np.uint32)
+    lib().orc_merge_nodes(_ptr(base), n, dim, _ptr(nodes), nodes.shape[0],
+                          _ptr(off), _ptr(srcs), cap, _ptr(pruned), _ptr(pcnt),
+                          pruned.shape[1], deg, 1 if strict else 0, _ptr(out))
     return out
 
 

@@ -47,7 +47,7 @@ class PgStats(ctypes.Structure):
 EXPORTED = (
     "pg_params_init", "pg_params_check", "pg_create", "pg_destroy",
     "pg_gpu_stage_async", "pg_gpu_stage_device_async", "pg_wait", "pg_query",
-    "pg_get_stats", "pg_reset_stats", "pg_last_error", "pg_stage_name",
+    "pg_get_stats", "pg_reset_stats", "pg_get_stage_counters", "pg_last_error", "pg_stage_name",
     "pg_run_match", "pg_run_prune", "pg_run_reverse", "pg_run_merge",
     "pg_run_search", "pg_sync",
 )
@@ -87,6 +87,8 @@ def _declare(L):
     for f in ("pg_wait", "pg_query", "pg_reset_stats", "pg_sync"):
         getattr(L, f).argtypes = [_P]
     L.pg_get_stats.argtypes = [_P, ctypes.POINTER(PgStats)]
+    L.pg_get_stage_counters.argtypes = [_P, _U32,
+                                        ctypes.POINTER(ctypes.c_uint64)]
     L.pg_last_error.restype = ctypes.c_char_p
     L.pg_last_error.argtypes = [_P]
     L.pg_stage_name.restype = ctypes.c_char_p
@@ -186,6 +188,12 @@ class Context:
         s = PgStats()
         _check(lib().pg_get_stats(self._ctx, ctypes.byref(s)), self._ctx)
         return s
+
+    def stage_counters(self, stage):
+        buf = (ctypes.c_uint64 * 16)()
+        _check(lib().pg_get_stage_counters(self._ctx, stage, buf), self._ctx)
+        names = [n for n, _ in PgStats._fields_[:9]]
+        return {n: int(buf[i]) for i, n in enumerate(names)}
 
     def reset_stats(self):
         _check(lib().pg_reset_stats(self._ctx), self._ctx)

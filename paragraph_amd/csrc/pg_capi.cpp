@@ -50,6 +50,7 @@ struct pg_ctx {
   uint32_t* srcs = nullptr;
   uint32_t* res = nullptr;
   unsigned long long* stats = nullptr;
+  unsigned long long* stage_cnt = nullptr;  // (kMaxStages+1) x 16 snapshots
   // staging for the host-pointer entry point (allocated on first use)
   float* h_base_dev = nullptr;
   uint32_t* h_gt_dev = nullptr;
@@ -104,7 +105,11 @@ uint32_t next_pow2(uint32_t x) {
 }
 
 int record(pg_ctx* c, int idx) {
-  if (idx <= kMaxStages) PG_HIP(c, hipEventRecord(c->ev[idx], c->stream));
+  if (idx <= kMaxStages) {
+    PG_HIP(c, hipEventRecord(c->ev[idx], c->stream));
+    PG_HIP(c, pg::launch_snapshot_stats(c->stats, c->stage_cnt + idx * 16,
+                                        c->stream));
+  }
   return PG_OK;
 }
 
@@ -292,7 +297,8 @@ pg_ctx* pg_create(int device, const pg_params* p) {
       dev_alloc(c, &c->scan_tmp, nb, "scan_tmp") ||
       dev_alloc(c, &c->srcs, N * c->mp_max, "srcs") ||
       dev_alloc(c, &c->res, N * p->topk, "search_out") ||
-      dev_alloc(c, &c->stats, kStatCount, "stats"))
+      dev_alloc(c, &c->stats, kStatCount, "stats") ||
+      dev_alloc(c, &c->stage_cnt, (kMaxStages + 1) * 16ull, "stage_cnt"))
     return fail(0);
   if (pg::launch_fill_u64(c->stats, kStatCount, 0ull, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -316,6 +322,7 @@ void pg_destroy(pg_ctx* c) {
   dev_free(c->srcs);
   dev_free(c->res);
   dev_free(c->stats);
+  dev_free(c->stage_cnt);
   dev_free(c->h_base_dev);
   dev_free(c->h_gt_dev);
   dev_free(c->h_proj_dev);
@@ -415,6 +422,18 @@ int pg_get_stats(pg_ctx* c, pg_stats* out) {
     if (hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]) == hipSuccess)
       out->stage_ms[i] = ms;
   }
+  return PG_OK;
+}
+
+int pg_get_stage_counters(pg_ctx* c, uint32_t stage, uint64_t* out16) {
+  if (!c || !out16) return set_err(c, PG_ERR_INVALID, "NULL argument");
+  if (c->n_stage_events < 2 || stage + 1 >= uint32_t(c->n_stage_events))
+    return set_err(c, PG_ERR_INVALID, "stage %u not recorded", stage);
+  PG_TRY(pg_wait(c));
+  unsigned long long rows[2][16];
+  PG_HIP(c, hipMemcpy(rows, c->stage_cnt + stage * 16, sizeof(rows),
+                      hipMemcpyDeviceToHost));
+  for (int k = 0; k < 16; ++k) out16[k] = rows[1][k] - rows[0][k];
   return PG_OK;
 }
 

@@ -9,6 +9,8 @@ namespace pg {
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n,
                            unsigned long long v, hipStream_t s);
+hipError_t launch_snapshot_stats(const unsigned long long* stats,
+                                 unsigned long long* row, hipStream_t s);
 hipError_t launch_match(const uint32_t* gt, uint32_t n_query, uint32_t gt_k,
                         uint32_t n_base, uint32_t* winner, hipStream_t s);
 hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,

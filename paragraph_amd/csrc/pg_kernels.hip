@@ -29,6 +29,26 @@ __global__ void fill_u64_kernel(unsigned long long* __restrict__ p, uint64_t n,
     p[i] = v;
 }
 
+// Sums the per-slot counters into one row (called between pipeline stages,
+// so that per-stage deltas can be reported).
+__global__ __launch_bounds__(256) void snapshot_stats_kernel(
+    const unsigned long long* __restrict__ stats,
+    unsigned long long* __restrict__ row) {
+  __shared__ unsigned long long part[256];
+  const uint32_t t = threadIdx.x;
+  const uint32_t f = t % kStatFields, s0 = t / kStatFields;  // 16 x 16
+  unsigned long long v = 0;
+  for (uint32_t s = s0; s < kStatSlots; s += 256 / kStatFields)
+    v += stats[s * kStatFields + f];
+  part[t] = v;
+  __syncthreads();
+  if (t < kStatFields) {
+    unsigned long long acc = 0;
+    for (uint32_t k = 0; k < 256 / kStatFields; ++k) acc += part[k * kStatFields + t];
+    row[t] = acc;
+  }
+}
+
 // ------------------------------------------------------------ K0: top-1 match
 // match_top1_kernel_v0 (Gbuilder.cuh:104-135).  The reference lets an
 // atomicCAS pick the owner of a contested base row; here the lowest query id
@@ -602,6 +622,12 @@ hipError_t launch_fill_u64(unsigned long long* p, uint64_t n,
                            unsigned long long v, hipStream_t s) {
   if (n == 0) return hipSuccess;
   fill_u64_kernel<<<grid_for(n, 256), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_snapshot_stats(const unsigned long long* stats,
+                                 unsigned long long* row, hipStream_t s) {
+  snapshot_stats_kernel<<<1, 256, 0, s>>>(stats, row);
   return hipGetLastError();
 }
 
