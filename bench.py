@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--n-eval", type=int, default=10000)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "
@@ -82,6 +84,14 @@ def main():
     proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
     link_d = torch.empty((n, M), dtype=torch.int32, device=dev)
     ctx = _native.Context(p)
+    if args.calib_copy:
+        # a plain 1 GiB device copy, the reference point for HBM counters
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for i in range(args.warmup):
         ctx.stage_device(base_d, gt_d, proj_d, link_d)
         ctx.wait()

@@ -1,0 +1,82 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point include/paragraph.h declares, validates parameters, and fails cleanly
+(no crash, no CPU fallback) when no device is present."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from paragraph_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "paragraph.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _native.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 18
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(_native.EXPORTED) <= set(syms)
+
+
+def test_params_init_mapping():
+    p = _native.make_params(1_000_000, 200, 1_000_000, 32, 128)
+    assert (p.M, p.link_rev, p.Km, p.Kd, p.Kp, p.topk) == (32, 94, 128, 16,
+                                                            8, 256)
+    assert (p.proj_deg, p.proj_pr, p.proj_rev) == (16, 15, 111)
+    assert (p.reset_iter, p.table_size, p.rounds, p.entry_seed) == (
+        15, 4096, 2, 3030517)
+    p = _native.make_params(100_000, 200, 100_000, 32, 100)
+    assert (p.Kp, p.topk) == (7, 200)
+    assert _native.check_params(p)[0] == 0
+
+
+@pytest.mark.parametrize("field,value,needle", [
+    ("n_base", 0, "n_base"),
+    ("n_base", 1 << 31, "n_base"),
+    ("dim", 0, "dim"),
+    ("M", 1, "M must"),
+    ("link_rev", 100, "link_rev"),
+    ("table_size", 3000, "power of two"),
+    ("table_size", 2048, "exceed"),
+    ("topk", 1000, "topk"),
+    ("Kd", 17, "Kd"),
+    ("reset_iter", 0, "reset_iter"),
+])
+def test_params_check_rejects(field, value, needle):
+    p = _native.make_params(1000, 64, 1000, 32, 128)
+    setattr(p, field, value)
+    rc, msg = _native.check_params(p)
+    assert rc == -1 and needle in msg, msg
+
+
+def test_create_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    p = _native.make_params(1000, 64, 1000, 32, 64)
+    with pytest.raises(_native.PgError):
+        _native.Context(p)
+
+
+def test_invalid_params_rejected_before_device():
+    p = _native.make_params(1000, 64, 1000, 32, 64, table_size=1000)
+    L = _native.lib()
+    ctx = L.pg_create(0, ctypes.byref(p))
+    assert not ctx
+    assert b"invalid params" in L.pg_last_error(None)
+
+
+def test_stage_names():
+    assert _native.stage_name(0) == "match"
+    assert _native.stage_name(4) == "search_round1"
+    assert _native.stage_name(11) == "merge_round2"

@@ -1,0 +1,368 @@
+"""Pins the CPU oracle (oracle/cpu_ref.cpp) without the reference binary.
+
+The reference publishes no golden vectors and cannot be built here, so each
+oracle function is checked against what the mathematics fixes:
+  - distances: exact-rational FMA emulation of the summation order, FP64
+    error bounds, and hand-derived inputs on which a different order or an
+    unfused multiply-add gives a different float;
+  - match / top-1 rows / reverse lists: closed forms and brute force;
+  - RNG prune / merge: hand-derived 2-D cases and an independent brute-force
+    restatement in Python with FP64 distances on well-separated points;
+  - search: a complete graph, where the result must be the exact top-Km.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import cpu_ref
+
+TOMB = 0xFFFFFFFF
+INT_MAX = 0x7FFFFFFF
+
+
+# ------------------------------------------------------------------ helpers
+
+def round_f32(fr):
+    """Round a rational to the nearest float32 (ties to even)."""
+    if fr == 0:
+        return np.float32(0.0)
+    f = np.float32(float(fr))
+    cands = [np.nextafter(f, np.float32(-np.inf)), f,
+             np.nextafter(f, np.float32(np.inf))]
+    best = min(cands, key=lambda c: (abs(Fraction(float(c)) - fr),
+                                     int(np.array(c).view(np.uint32)) & 1))
+    return np.float32(best)
+
+
+def fma32(a, b, c):
+    return round_f32(Fraction(float(a)) * Fraction(float(b)) +
+                     Fraction(float(c)))
+
+
+def chain_ref(x, y):
+    s = np.float32(0.0)
+    for a, b in zip(x, y):
+        s = fma32(a, b, s)
+    return np.float32(-s)
+
+
+def search_ref(q, x):
+    p = []
+    for l in range(32):
+        s = np.float32(0.0)
+        for i in range(l, len(q), 32):
+            s = fma32(q[i], x[i], s)
+        p.append(s)
+    o = 16
+    while o:
+        for l in range(o):
+            p[l] = np.float32(p[l] + p[l + o])
+        o >>= 1
+    return np.float32(-p[0])
+
+
+def gamma(n):
+    u = 2.0 ** -24
+    return n * u / (1 - n * u)
+
+
+# ------------------------------------------------------------- distances
+
+def test_chain_dist_matches_exact_fma_emulation():
+    rng = np.random.default_rng(1)
+    for dim in (1, 7, 64, 200):
+        for _ in range(3):
+            x = rng.standard_normal(dim).astype(np.float32)
+            y = rng.standard_normal(dim).astype(np.float32)
+            assert cpu_ref.chain_dist(x, y) == chain_ref(x, y)
+
+
+def test_search_dist_matches_exact_fma_emulation():
+    rng = np.random.default_rng(2)
+    for dim in (1, 31, 33, 200, 512):
+        q = rng.standard_normal(dim).astype(np.float32)
+        x = rng.standard_normal(dim).astype(np.float32)
+        assert cpu_ref.search_dist(q, x) == search_ref(q, x)
+
+
+def test_distances_within_fp64_error_bound():
+    rng = np.random.default_rng(3)
+    for dim in (200, 512):
+        for _ in range(20):
+            x = rng.standard_normal(dim).astype(np.float32)
+            y = rng.standard_normal(dim).astype(np.float32)
+            exact = -float(np.dot(x.astype(np.float64), y.astype(np.float64)))
+            bound = gamma(dim) * float(np.abs(x.astype(np.float64) *
+                                              y.astype(np.float64)).sum())
+            assert abs(cpu_ref.chain_dist(x, y) - exact) <= bound
+            assert abs(cpu_ref.search_dist(x, y) - exact) <= bound
+
+
+def test_chain_dist_is_fused():
+    # s0 = -(1+2^-11); s1 = fma(a, a, s0) with a = 1+2^-12: exact 2^-24 when
+    # fused, 0 when a*a is rounded first.
+    a = np.float32(1 + 2.0 ** -12)
+    x = np.array([1.0, a], np.float32)
+    y = np.array([-(1 + 2.0 ** -11), a], np.float32)
+    assert cpu_ref.chain_dist(x, y) == np.float32(-(2.0 ** -24))
+
+
+def test_chain_dist_is_sequential_from_index_zero():
+    # 1e8 + 1 rounds back to 1e8 in float32, so the sequential sum is 0;
+    # any order that adds 1 last (or pairs -1e8 with 1e8 first) gives 1.
+    x = np.array([1e8, 1.0, -1e8], np.float32)
+    y = np.ones(3, np.float32)
+    assert cpu_ref.chain_dist(x, y) == 0.0
+    x2 = np.array([1e8, -1e8, 1.0], np.float32)  # cancels first, keeps the 1
+    assert cpu_ref.chain_dist(x2, y) == np.float32(-1.0)
+
+
+def test_search_dist_groups_by_lane_stride_32():
+    # lane 0 holds elements 0 and 32 (1e8 - 1e8 = 0), lane 1 holds 1.0:
+    # the tree gives 1; a sequential chain would give 0.
+    q = np.zeros(64, np.float32)
+    q[0], q[1], q[32] = 1e8, 1.0, -1e8
+    x = np.ones(64, np.float32)
+    assert cpu_ref.search_dist(q, x) == np.float32(-1.0)
+    assert cpu_ref.chain_dist(q, x) == 0.0
+
+
+def test_search_tree_order():
+    # lanes 0 and 16 meet first: (1e8 + -1e8) + 1 = 1; adding lane 1 before
+    # lane 16 would lose the 1.
+    q = np.zeros(32, np.float32)
+    q[0], q[16], q[1] = 1e8, -1e8, 1.0
+    assert cpu_ref.search_dist(q, np.ones(32, np.float32)) == np.float32(-1.0)
+
+
+# ------------------------------------------------------ match / top-1 rows
+
+def test_match_winner_lowest_query_wins():
+    gt = np.array([[3, 1, 2], [0, 2, 1], [3, 0, 1], [TOMB, 1, 2], [7, 1, 2],
+                   [0, 1, 3]], np.uint32)
+    w = cpu_ref.match_winner(gt, 5)
+    assert w.tolist() == [1, TOMB, TOMB, 0, TOMB]
+
+
+def test_match_winner_brute_force():
+    rng = np.random.default_rng(4)
+    n, q = 50, 400
+    gt = rng.integers(0, n, (q, 8)).astype(np.uint32)
+    w = cpu_ref.match_winner(gt, n)
+    for b in range(n):
+        qs = np.nonzero(gt[:, 0] == b)[0]
+        assert w[b] == (qs.min() if qs.size else TOMB)
+
+
+def test_top1_rows_layout():
+    gt = np.arange(2 * 128, dtype=np.uint32).reshape(2, 128)
+    w = np.array([1, TOMB, 0], np.uint32)
+    rows = cpu_ref.top1_rows(gt, w)
+    assert rows[0, :126].tolist() == list(range(129, 255))
+    assert rows[0, 126:].tolist() == [TOMB, TOMB]
+    assert (rows[1] == TOMB).all()
+    assert rows[2, :126].tolist() == list(range(1, 127))
+
+
+# ------------------------------------------------------------- RNG prune
+
+def rng_prune_fp64(base, b, cand, mp):
+    """Independent restatement with FP64 distances (valid when no pair of
+    compared distances is within float32 rounding of each other)."""
+    X = base.astype(np.float64)
+    n = len(base)
+    d = []
+    for j, c in enumerate(cand):
+        if c in (TOMB, INT_MAX) or c == b or c >= n:
+            d.append(np.inf)
+        else:
+            d.append(-X[b] @ X[c])
+    order = sorted(range(len(cand)), key=lambda j: (d[j], j))
+    first = cand[order[0]]
+    if first in (TOMB, INT_MAX) or first >= n:
+        return []
+    kept = [first]
+    for j in order[1:]:
+        c = cand[j]
+        if c in (TOMB, INT_MAX) or c >= n:
+            break
+        if len(kept) >= mp:
+            break
+        dc = d[j]
+        if not any(-X[c] @ X[k] < dc for k in kept):
+            kept.append(c)
+    return kept
+
+
+def test_prune_hand_case_2d():
+    # node 0 at origin-ish direction; IP distance = -<b, c>.
+    base = np.array([[1.0, 0.0],    # 0: the node
+                     [3.0, 0.0],    # 1: closest (d=-3)
+                     [2.0, 2.0],    # 2: d=-2, occluded by 1? -<2,1>=-6 < -2
+                     [0.0, 1.0],    # 3: d=0, -<3,1>=0 < 0? no -> kept
+                     [-1.0, 0.0]],  # 4: d=1, -<4,1>=3, -<4,3>=0 < 1 -> occl
+                    np.float32)
+    rows = np.full((5, 6), TOMB, np.uint32)
+    rows[0, :5] = [4, 3, 2, 1, 0]
+    pr, pc = cpu_ref.prune(base, rows, 4)
+    assert pc[0] == 2
+    assert pr[0, :2].tolist() == [1, 3]
+    assert pc[1:].tolist() == [0, 0, 0, 0]  # all-tomb rows are skipped
+
+
+def test_prune_stop_ids_and_cap():
+    base = np.random.default_rng(5).standard_normal((6, 3)).astype(np.float32)
+    rows = np.full((6, 8), TOMB, np.uint32)
+    rows[2] = [INT_MAX, 2, 100, TOMB, 1, 3, 4, 5]  # INT_MAX / self / >= n
+    pr, pc = cpu_ref.prune(base, rows, 3)
+    ref = rng_prune_fp64(base, 2, rows[2].tolist(), 3)
+    assert pc[2] == len(ref) and pr[2, :pc[2]].tolist() == ref
+    assert pc[2] <= 3
+
+
+def test_prune_brute_force_random():
+    rng = np.random.default_rng(6)
+    for trial in range(30):
+        n, dim = 12, 4
+        base = (rng.standard_normal((n, dim)) * 3).astype(np.float32)
+        rows = rng.integers(0, n, (n, 10)).astype(np.uint32)
+        rows[rng.random((n, 10)) < 0.1] = TOMB
+        mp = int(rng.integers(2, 6))
+        pr, pc = cpu_ref.prune(base, rows, mp)
+        for b in range(n):
+            ref = rng_prune_fp64(base, b, rows[b].tolist(), mp)
+            assert pr[b, :pc[b]].tolist() == ref, (trial, b)
+
+
+# ------------------------------------------------------ reverse / merge
+
+def test_reverse_csr_invariants():
+    rng = np.random.default_rng(7)
+    n, mp = 40, 5
+    pr = rng.integers(0, n, (n, mp)).astype(np.uint32)
+    pc = rng.integers(0, mp + 1, n).astype(np.uint32)
+    off, srcs = cpu_ref.reverse_csr(pr, pc)
+    assert off[-1] == pc.sum()
+    for c in range(n):
+        seg = srcs[off[c]:off[c + 1]].tolist()
+        assert seg == sorted(seg)
+        expect = sorted(b for b in range(n) for j in range(pc[b])
+                        if pr[b, j] == c)
+        assert seg == expect
+
+
+def test_merge_non_strict_fill_and_duplicates():
+    # node 0: reverse list {2}, pruned list {1, 2}: the duplicate 2 appears
+    # twice; strict RNG keeps 1 then tests 2 twice; non-strict fill appends
+    # the unselected entries in sorted order, marking only the first copy.
+    base = np.array([[1.0, 0.0], [2.0, 0.0], [0.0, 1.0], [5.0, 5.0]],
+                    np.float32)
+    pr = np.full((4, 3), TOMB, np.uint32)
+    pc = np.zeros(4, np.uint32)
+    pr[0, :2] = [1, 2]
+    pc[0] = 2
+    off = np.array([0, 1, 1, 1, 1], np.uint32)
+    srcs = np.array([2], np.uint32)
+    out_s = cpu_ref.merge(base, off, srcs, 3, pr, pc, 4, strict=True)
+    out_n = cpu_ref.merge(base, off, srcs, 3, pr, pc, 4, strict=False)
+    # sorted: 1 (d=-2), 2 (d=0, rev copy), 2 (d=0, pruned copy)
+    # -<2,1> = 0 < 0? no -> 2 kept; second copy: -<2,2> = -1 < 0 -> occluded
+    assert out_s[0].tolist() == [1, 2, TOMB, TOMB]
+    # fill: marks list slot of 1 and the FIRST slot of 2; appends the second
+    # copy of 2
+    assert out_n[0].tolist() == [1, 2, 2, TOMB]
+    assert (out_s[1:] == TOMB).all()
+
+
+def test_merge_cap_keeps_smallest_sources():
+    base = np.random.default_rng(8).standard_normal((10, 3)).astype(np.float32)
+    pr = np.full((10, 2), TOMB, np.uint32)
+    pc = np.zeros(10, np.uint32)
+    for b in range(1, 10):
+        pr[b, 0] = 0
+        pc[b] = 1
+    off, srcs = cpu_ref.reverse_csr(pr, pc)
+    assert srcs[off[0]:off[1]].tolist() == list(range(1, 10))
+    st = cpu_ref.OrcStats()
+    out = cpu_ref.merge(base, off, srcs, 3, pr, pc, 5, True, stats=st)
+    # only sources 1, 2, 3 are candidates of node 0
+    assert set(out[0][out[0] != TOMB].tolist()) <= {1, 2, 3}
+    assert st.rev_overflow == 1 and st.merge_dist == 3 + 9
+
+
+# -------------------------------------------------------------- search
+
+def complete_graph(n, deg):
+    g = np.full((n, deg), TOMB, np.uint32)
+    for v in range(n):
+        others = [u for u in range(n) if u != v]
+        g[v, :len(others)] = others
+    return g
+
+
+def test_search_complete_graph_is_exact_topk():
+    rng = np.random.default_rng(9)
+    n, dim = 20, 16
+    base = rng.standard_normal((n, dim)).astype(np.float32)
+    qs = rng.standard_normal((5, dim)).astype(np.float32)
+    g = complete_graph(n, n - 1)
+    Km, Kp, Kd, topk = 8, 1, n - 1, 8
+    out = cpu_ref.search(qs, base, g, Km, Kp, Kd, topk, entry_seed=0)
+    for q, row in zip(qs, out):
+        d = np.array([cpu_ref.search_dist(q, base[i]) for i in range(n)])
+        expect = sorted(range(n), key=lambda i: (d[i], i))[:Km]
+        assert row.tolist() == expect
+
+
+def test_search_output_sorted_and_masked():
+    rng = np.random.default_rng(10)
+    n, dim = 300, 8
+    base = rng.standard_normal((n, dim)).astype(np.float32)
+    g = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    g[rng.random((n, 16)) < 0.2] = TOMB
+    Km, Kp, Kd, topk = 16, 2, 16, 40
+    out = cpu_ref.search(base[:30], base, g, Km, Kp, Kd, topk)
+    assert (out < 0x80000000).all()
+    for q in range(30):
+        row = out[q, :Km]
+        assert (row != INT_MAX).all()
+        d = [cpu_ref.search_dist(base[q], base[i]) for i in row]
+        assert all(d[i] <= d[i + 1] for i in range(Km - 1))
+
+
+def test_search_stats_and_watchdog():
+    rng = np.random.default_rng(11)
+    n, dim = 200, 8
+    base = rng.standard_normal((n, dim)).astype(np.float32)
+    g = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    st = cpu_ref.OrcStats()
+    cpu_ref.search(base[:10], base, g, 16, 2, 16, 32, stats=st)
+    assert st.search_iters >= 10 * (16 // 2)
+    assert st.search_capped == 0
+    st2 = cpu_ref.OrcStats()
+    cpu_ref.search(base[:10], base, g, 16, 2, 16, 32, max_iters=2, stats=st2)
+    assert st2.search_capped == 10 and st2.search_iters == 20
+
+
+# ----------------------------------------------------- whole-stage shape
+
+def test_gpu_stage_invariants_small():
+    from paragraph_amd import datagen
+    n, dim = 400, 16
+    w = datagen.make_workload(n, dim, 8, device="cpu")
+    gt = np.ascontiguousarray(w["gt"].view(np.uint32))
+    params = dict(n_base=n, dim=dim, n_query=n, gt_k=128, proj_deg=16,
+                  proj_pr=15, proj_rev=111, M=32, link_rev=94, Km=32, Kp=2,
+                  Kd=16, topk=64, reset_iter=15, rounds=2,
+                  entry_seed=3030517, max_iters=100000)
+    st = cpu_ref.OrcStats()
+    proj, link = cpu_ref.gpu_stage(params, w["base"], gt, stats=st)
+    for g, deg in ((proj, 16), (link, 32)):
+        assert g.shape == (n, deg)
+        valid = g != TOMB
+        assert (g[valid] < n).all()
+        # rows are packed: valid entries first, then tombs
+        assert (np.diff(valid.astype(np.int8), axis=1) <= 0).all()
+    assert (link[:, 0] != TOMB).mean() > 0.9
+    assert st.search_capped == 0
