@@ -30,6 +30,21 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel, workload, precision):
+    """HBM bytes per launch recorded from rocprofv3 FETCH_SIZE/WRITE_SIZE
+    runs (scaled by a 1 GiB copy) in profiles/*.md, or None."""
+    import glob
+    import re
+    key = f"traffic_bytes_per_launch[{kernel},{workload},{precision}]"
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            if line.startswith(key):
+                m = re.search(r"=\s*([0-9]+)", line)
+                if m:
+                    return int(m.group(1))
+    return None
+
+
 def stage_bytes(kind, c, p, n, D=None, mp=None, deg=None):
     """Algorithmic HBM bytes of one launch from its counters (FP32 rows)."""
     row = 4 * p.dim
@@ -56,6 +71,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--n-eval", type=int, default=10000)
+    ap.add_argument("--tune-flags", type=int, default=0,
+                    help="perf-only knobs passed to pg_params.tune_flags")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
@@ -78,7 +95,7 @@ def main():
     log(f"workload {args.config} n={n} dim={dim} M={M} L={L} generated in "
         f"{time.time() - t0:.1f}s")
     gt_u32 = np.ascontiguousarray(w["gt"].view(np.uint32))
-    p = _native.make_params(n, dim, n, M, L)
+    p = _native.make_params(n, dim, n, M, L, tune_flags=args.tune_flags)
     base_d = torch.from_numpy(w["base"]).to(dev)
     gt_d = torch.from_numpy(w["gt"]).to(dev)
     proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
@@ -131,17 +148,20 @@ def main():
     avg_ms = kd["ms"] / kd["launches"]
     bytes_per_launch = kd["bytes"] / kd["launches"]
     achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
+    kname = {"search": "search_kernel", "prune": "prune_kernel",
+             "merge": "merge_kernel", "reverse": "reverse_*"}.get(dom, dom)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": {"search": "search_kernel", "prune": "prune_kernel",
-                           "merge": "merge_kernel",
-                           "reverse": "reverse_*"}.get(dom, dom),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(kname, args.config,
+                                            args.precision),
+                "kernel": kname,
                 "avg_launch_ms": round(avg_ms, 3),
                 "alg_bytes_per_launch": int(bytes_per_launch)}
     log("stage ms:", {nm: round(x, 2) for nm, x in zip(names, stage_ms)})
 
     extra = {"stage_ms": {nm: round(x, 3) for nm, x in zip(names, stage_ms)},
+             "stage_counters": {nm: c for nm, c in zip(names, counters)},
              "counters": st.as_dict()}
     extra["counters"].pop("stage_ms", None)
 

@@ -63,13 +63,18 @@ typedef struct pg_params {
   uint32_t Kd;          /* neighbours read per expanded node */
   uint32_t topk;        /* search output width (2L) */
   uint32_t reset_iter;  /* visited-table reset period (15) */
-  uint32_t table_size;  /* visited-table slots (power of two) */
+  uint32_t table_size;  /* visited-table slots in LDS; 0 = auto (the most ids
+                           one reset period can insert, Km + reset_iter*Kp*Kd,
+                           plus 1/16 slack).  The reference uses 4096
+                           (tests/test_t2iend.cu:694); the size changes speed
+                           only, never results. */
   uint32_t rounds;      /* link rounds (2) */
   uint32_t entry_seed;  /* entry points (seed + i) mod N (3030517) */
   uint32_t max_iters;   /* search iteration watchdog (not in the reference) */
   uint32_t precision;   /* pg_precision */
   uint32_t rerank;      /* screened paths: FP32 re-rank width (0 = topk) */
-  uint32_t tune_flags;  /* perf-only knobs, 0 = defaults / autotune */
+  uint32_t tune_flags;  /* perf-only knobs, 0 = defaults; bits 0-4: search
+                           gather batch (candidate pairs in flight) */
   uint32_t reserved[7];
 } pg_params;
 
@@ -94,7 +99,7 @@ typedef struct pg_ctx pg_ctx;
 
 /* Fills *p with the project's mapping of (N, dim, M, L) onto the reference
  * knobs: link_rev = 126-M, Km = L, Kd = 16, Kp = ceil(L/16), topk = 2L,
- * reset_iter 15, table 4096, 2 rounds, seed 3030517, projection 16/15/111
+ * reset_iter 15, table auto, 2 rounds, seed 3030517, projection 16/15/111
  * (reference values: tests/test_t2iend.cu:676-707). */
 int pg_params_init(pg_params* p, uint32_t n_base, uint32_t dim,
                    uint32_t n_query, uint32_t M, uint32_t L);

@@ -98,6 +98,20 @@ void dev_free(T*& p) {
   p = nullptr;
 }
 
+// Visited-table slots actually used: table_size, or (0 = auto) the most ids
+// one reset period can insert plus 1/16 slack, rounded up to 64 slots.
+uint32_t effective_table_size(const pg_params& p) {
+  if (p.table_size) return p.table_size;
+  const uint64_t bound = uint64_t(p.Km) + uint64_t(p.reset_iter) * p.Kp * p.Kd;
+  uint64_t t = bound + (bound / 16 > 64 ? bound / 16 : 64);
+  t = (t + 63) / 64 * 64;
+  return t > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(t);
+}
+
+// Perf-only knobs packed in tune_flags (0 = library default):
+//   bits 0-4  search gather batch (candidate pairs in flight; 4, 8 or 16)
+uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
+
 uint32_t next_pow2(uint32_t x) {
   uint32_t v = 1;
   while (v < x) v <<= 1;
@@ -144,7 +158,8 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   for (uint32_t r = 0; r < p.rounds; ++r) {
     PG_HIP(c, pg::launch_search(d_base, N, d_base, N, p.dim, graph, gdeg, p.Km,
                                 p.Kp, p.Kd, p.topk, p.reset_iter,
-                                p.entry_seed, p.max_iters, p.table_size,
+                                p.entry_seed, p.max_iters,
+                                effective_table_size(p), tune_gather(p),
                                 c->res, c->stats, s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, p.topk, nullptr,
@@ -188,7 +203,7 @@ int pg_params_init(pg_params* p, uint32_t n_base, uint32_t dim,
   p->Kp = (L + 15) / 16;
   p->topk = 2 * L;
   p->reset_iter = 15;
-  p->table_size = 4096;
+  p->table_size = 0;  // auto: see effective_table_size()
   p->rounds = 2;
   p->entry_seed = 3030517;
   p->max_iters = 100000;
@@ -223,16 +238,17 @@ int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len) {
   if (p->topk > 512) return fail("topk must be <= 512");
   if (p->Km + p->Kp * p->Kd > 4096) return fail("Km + Kp*Kd must be <= 4096");
   if (p->reset_iter == 0) return fail("reset_iter must be > 0");
-  if (p->table_size == 0 || (p->table_size & (p->table_size - 1)) != 0)
-    return fail("table_size must be a power of two");
-  if (uint64_t(p->table_size) <=
+  if (p->Kp * p->Kd < p->Km)
+    return fail("Kp*Kd must be >= Km (the entry points fill the first list)");
+  if (uint64_t(effective_table_size(*p)) <=
       uint64_t(p->Km) + uint64_t(p->reset_iter) * p->Kp * p->Kd)
     return fail("table_size must exceed Km + reset_iter*Kp*Kd (else the "
                 "visited table can fill up)");
   {
     uint32_t lp2 = 64;
     while (lp2 < p->Km + p->Kp * p->Kd) lp2 <<= 1;
-    if (pg::search_smem_bytes(p->table_size, lp2, p->Kp * p->Kd, p->Kp) >
+    if (pg::search_smem_bytes(effective_table_size(*p), lp2, p->Kp * p->Kd,
+                              p->Kp) >
         160u * 1024u)
       return fail("search LDS footprint exceeds 160 KiB");
   }
@@ -512,7 +528,8 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
   PG_HIP(c, pg::launch_search(d_queries, n_query, d_base, p.n_base, p.dim,
                               d_graph, graph_deg, p.Km, p.Kp, p.Kd, p.topk,
                               p.reset_iter, p.entry_seed, p.max_iters,
-                              p.table_size, d_out, c->stats, c->stream));
+                              effective_table_size(p), tune_gather(p), d_out,
+                              c->stats, c->stream));
   return PG_OK;
 }
 

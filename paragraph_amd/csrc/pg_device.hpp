@@ -112,6 +112,75 @@ __device__ void bitonic_sort(K* a, uint32_t n) {
   }
 }
 
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, uint32_t m) {
+  const uint32_t lo = __shfl_xor(uint32_t(v), int(m), 64);
+  const uint32_t hi = __shfl_xor(uint32_t(v >> 32), int(m), 64);
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Ascending bitonic sort of 64*R keys held in registers, element index
+// lane*R + r.  Cross-lane steps exchange through __shfl_xor (no LDS).
+template <int R>
+__device__ __forceinline__ void reg_bitonic_sort(uint64_t (&v)[R]) {
+  const uint32_t lane = lane_id();
+  constexpr uint32_t n = 64u * R;
+#pragma unroll
+  for (uint32_t k = 2; k <= n; k <<= 1) {
+#pragma unroll
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j >= uint32_t(R)) {
+        const uint32_t lj = j / R;
+        const bool lower = (lane & lj) == 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const uint32_t e = lane * R + r;
+          const bool asc = (e & k) == 0;
+          const uint64_t o = shfl_xor_u64(v[r], lj);
+          const uint64_t mn = v[r] < o ? v[r] : o;
+          const uint64_t mx = v[r] < o ? o : v[r];
+          v[r] = (lower == asc) ? mn : mx;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if ((uint32_t(r) & j) == 0) {
+            const uint32_t e = lane * R + r;
+            const bool asc = (e & k) == 0;
+            const uint64_t a = v[r], b = v[r + j];
+            if ((a > b) == asc) {
+              v[r] = b;
+              v[r + j] = a;
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Number of entries of the sorted LDS array a[0..n) that are < x
+// (lower bound) or <= x (upper bound).
+__device__ __forceinline__ uint32_t count_less(const uint64_t* a, uint32_t n,
+                                               uint64_t x) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ uint32_t count_less_eq(const uint64_t* a,
+                                                  uint32_t n, uint64_t x) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __device__ __forceinline__ void stat_add(unsigned long long* stats, int field,
                                          unsigned long long v) {
   if (stats != nullptr && v != 0)

@@ -36,7 +36,7 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t Kp, uint32_t Kd, uint32_t topk,
                          uint32_t reset_iter, uint32_t entry_seed,
                          uint32_t max_iters, uint32_t table_size,
-                         uint32_t* out, unsigned long long* stats,
-                         hipStream_t s);
+                         uint32_t gather_batch, uint32_t* out,
+                         unsigned long long* stats, hipStream_t s);
 
 }  // namespace pg

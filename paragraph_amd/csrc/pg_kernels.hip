@@ -430,19 +430,23 @@ __global__ __launch_bounds__(64) void merge_kernel(
 // lanes) per candidate, lane l accumulating elements l, l+32, ... with the
 // query slice held in registers, then the xor tree 16/8/4/2/1, whose lane-0
 // result equals the reference's __shfl_down_sync tree.
-__device__ __forceinline__ uint32_t visit_hash(uint32_t id, uint32_t mask) {
-  return (id * 2654435761u) & mask;
+// Visited set: open addressing with linear probing over `tsize` slots (any
+// size; the host sizes it above the most ids one reset period can insert,
+// Km + reset_iter*Kp*Kd, so a probe always ends).  Slot 0 = empty, else
+// id + 1.  Multiplicative hash, range-reduced by multiply-shift.
+__device__ __forceinline__ uint32_t visit_hash(uint32_t id, uint32_t tsize) {
+  return uint32_t((uint64_t(id * 2654435761u) * tsize) >> 32);
 }
 
-__device__ __forceinline__ bool visit_insert(uint32_t* tab, uint32_t mask,
+__device__ __forceinline__ bool visit_insert(uint32_t* tab, uint32_t tsize,
                                              uint32_t id) {
   const uint32_t key = id + 1u;
-  uint32_t s = visit_hash(id, mask);
+  uint32_t s = visit_hash(id, tsize);
   while (true) {
     const uint32_t old = atomicCAS(&tab[s], 0u, key);
     if (old == 0u) return true;
     if (old == key) return false;
-    s = (s + 1u) & mask;
+    s = (s + 1u == tsize) ? 0u : s + 1u;
   }
 }
 
@@ -486,7 +490,68 @@ __device__ __forceinline__ void eval_misses(
   }
 }
 
-template <int TQ>
+// Merges the sorted misses of one iteration into the top-Km list (the
+// reference re-sorts the whole list, Gsearcher.cuh:1045; only its first Km
+// entries survive into the next iteration, so a merge is equivalent).
+// Returns true when this iteration is the last one (no unexpanded entry
+// would stay in the top Km, or the watchdog stops after it); the caller then
+// sorts the full list once, which also orders the tail the output needs.
+template <int R>
+__device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
+                             const uint32_t* todo, uint32_t m, uint64_t* ms,
+                             bool force_final) {
+  const uint32_t lane = lane_id();
+  uint64_t mv[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t idx = lane * R + r;
+    mv[r] = idx < m ? tail[todo[idx]] : kPadKey;
+  }
+  reg_bitonic_sort<R>(mv);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t idx = lane * R + r;
+    if (idx < m) ms[idx] = mv[r];
+  }
+  __syncthreads();
+  constexpr int kHpl = 4;  // Km <= 256
+  uint64_t hx[kHpl];
+  uint32_t hp[kHpl];
+  bool unexp = false;
+#pragma unroll
+  for (int s = 0; s < kHpl; ++s) {
+    const uint32_t i = lane + 64u * s;
+    hx[s] = 0;
+    hp[s] = 0xFFFFFFFFu;
+    if (i < Km) {
+      hx[s] = keys[i];
+      hp[s] = i + count_less(ms, m, hx[s]);
+      if (hp[s] < Km && (hx[s] & 1ull) == 0) unexp = true;
+    }
+  }
+  uint32_t mpos[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t idx = lane * R + r;
+    mpos[r] = 0xFFFFFFFFu;
+    if (idx < m) {
+      mpos[r] = idx + count_less_eq(keys, Km, mv[r]);
+      if (mpos[r] < Km) unexp = true;
+    }
+  }
+  if (force_final || __ballot(unexp) == 0) return true;
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kHpl; ++s)
+    if (hp[s] < Km) keys[hp[s]] = hx[s];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (mpos[r] < Km) keys[mpos[r]] = mv[r];
+  __syncthreads();
+  return false;
+}
+
+template <int TQ, int U>
 __global__ __launch_bounds__(64) void search_kernel(
     const float* __restrict__ queries, uint32_t n_query,
     const float* __restrict__ base, uint32_t n_base, uint32_t dim,
@@ -497,14 +562,16 @@ __global__ __launch_bounds__(64) void search_kernel(
   extern __shared__ __align__(16) uint8_t smem[];
   const uint32_t kpkd = Kp * Kd;
   uint32_t* tab = reinterpret_cast<uint32_t*>(smem);
-  uint64_t* keys = reinterpret_cast<uint64_t*>(smem + size_t(tsize) * 4);
-  uint32_t* cand = reinterpret_cast<uint32_t*>(keys + lp2);
+  uint64_t* keys =
+      reinterpret_cast<uint64_t*>(smem + ((size_t(tsize) * 4 + 15) & ~size_t(15)));
+  uint64_t* ms = keys + lp2;
+  uint32_t* cand = reinterpret_cast<uint32_t*>(ms + kpkd);
   uint32_t* todo = cand + kpkd;
   uint32_t* explore = todo + kpkd;
   const uint32_t q = blockIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t l = lane & 31u;
-  const uint32_t tmask = tsize - 1u;
+  const bool fast_merge = Km <= 256 && kpkd <= 256;
   float qv[TQ];
 #pragma unroll
   for (int t = 0; t < TQ; ++t) {
@@ -520,7 +587,7 @@ __global__ __launch_bounds__(64) void search_kernel(
   }
   __syncthreads();
   uint64_t* tail = keys + Km;
-  eval_misses<TQ, 4>(qv, base, dim, cand, todo, kpkd, tail);
+  eval_misses<TQ, U>(qv, base, dim, cand, todo, kpkd, tail);
   unsigned long long ndist = kpkd;
   __syncthreads();
   bitonic_sort(keys, lp2);
@@ -528,7 +595,7 @@ __global__ __launch_bounds__(64) void search_kernel(
     for (uint32_t i = lane; i < tsize; i += kWave) tab[i] = 0u;
     __syncthreads();
     for (uint32_t i = lane; i < Km; i += kWave)
-      visit_insert(tab, tmask, key_raw_id(keys[i]) & kMaskedTomb);
+      visit_insert(tab, tsize, key_raw_id(keys[i]) & kMaskedTomb);
     __syncthreads();
   };
   reset_table();
@@ -575,7 +642,7 @@ __global__ __launch_bounds__(64) void search_kernel(
       bool miss = false;
       if (t < kpkd) {
         const uint32_t id = cand[t];
-        if (id != kTomb) miss = visit_insert(tab, tmask, id);
+        if (id != kTomb) miss = visit_insert(tab, tsize, id);
         if (!miss) tail[t] = search_key(FLT_MAX, id);
       }
       const uint64_t m = __ballot(miss);
@@ -583,10 +650,20 @@ __global__ __launch_bounds__(64) void search_kernel(
       ntodo += __popcll(m);
     }
     __syncthreads();
-    eval_misses<TQ, 4>(qv, base, dim, cand, todo, ntodo, tail);
+    eval_misses<TQ, U>(qv, base, dim, cand, todo, ntodo, tail);
     ndist += ntodo;
     __syncthreads();
-    bitonic_sort(keys, lp2);
+    const bool last_allowed = total_iter + 1 >= max_iters;
+    bool final_sort = true;
+    if (fast_merge) {
+      if (ntodo <= 64)
+        final_sort = merge_misses<1>(keys, Km, tail, todo, ntodo, ms, last_allowed);
+      else if (ntodo <= 128)
+        final_sort = merge_misses<2>(keys, Km, tail, todo, ntodo, ms, last_allowed);
+      else
+        final_sort = merge_misses<4>(keys, Km, tail, todo, ntodo, ms, last_allowed);
+    }
+    if (final_sort) bitonic_sort(keys, lp2);
     ++total_iter;
     if (++iter == reset_iter) {
       reset_table();
@@ -695,8 +772,8 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
 
 size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
                          uint32_t Kp) {
-  return size_t(table_size) * 4 + size_t(lp2) * 8 + size_t(kpkd) * 8 +
-         size_t(Kp) * 4 + 16;
+  return ((size_t(table_size) * 4 + 15) & ~size_t(15)) + size_t(lp2) * 8 +
+         size_t(kpkd) * 16 + size_t(Kp) * 4 + 16;
 }
 
 hipError_t launch_search(const float* queries, uint32_t n_query,
@@ -705,27 +782,38 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t Kp, uint32_t Kd, uint32_t topk,
                          uint32_t reset_iter, uint32_t entry_seed,
                          uint32_t max_iters, uint32_t table_size,
-                         uint32_t* out, unsigned long long* stats,
-                         hipStream_t s) {
+                         uint32_t gather_batch, uint32_t* out,
+                         unsigned long long* stats, hipStream_t s) {
   if (n_query == 0) return hipSuccess;
   uint32_t lp2 = 64;
   while (lp2 < Km + Kp * Kd) lp2 <<= 1;
   const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp);
   const uint32_t tq = (dim + 31) / 32;
-#define PG_SEARCH(T)                                                          \
+  // gather_batch: candidate pairs whose rows are in flight together
+  // (perf only); 0 picks by the query slice size.
+  uint32_t u = gather_batch;
+  if (u == 0) u = 4;
+#define PG_SEARCH(T, UU)                                                      \
   do {                                                                        \
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&search_kernel<T>), \
-                        hipFuncAttributeMaxDynamicSharedMemorySize,           \
-                        int(smem));                                           \
-    search_kernel<T><<<n_query, 64, smem, s>>>(                               \
+    (void)hipFuncSetAttribute(                                                \
+        reinterpret_cast<const void*>(&search_kernel<T, UU>),                 \
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));               \
+    search_kernel<T, UU><<<n_query, 64, smem, s>>>(                           \
         queries, n_query, base, n_base, dim, graph, gdeg, Km, Kp, Kd, topk,   \
         reset_iter, entry_seed, max_iters, table_size, lp2, out, stats);      \
   } while (0)
-  if (tq <= 4) PG_SEARCH(4);
-  else if (tq <= 8) PG_SEARCH(8);
-  else if (tq <= 16) PG_SEARCH(16);
-  else if (tq <= 32) PG_SEARCH(32);
+#define PG_SEARCH_U(T)                 \
+  do {                                 \
+    if (u <= 4) PG_SEARCH(T, 4);       \
+    else if (u <= 8) PG_SEARCH(T, 8);  \
+    else PG_SEARCH(T, 16);             \
+  } while (0)
+  if (tq <= 4) PG_SEARCH_U(4);
+  else if (tq <= 8) PG_SEARCH_U(8);
+  else if (tq <= 16) PG_SEARCH_U(16);
+  else if (tq <= 32) PG_SEARCH(32, 4);
   else return hipErrorInvalidValue;
+#undef PG_SEARCH_U
 #undef PG_SEARCH
   return hipGetLastError();
 }

@@ -34,7 +34,7 @@ def test_params_init_mapping():
                                                             8, 256)
     assert (p.proj_deg, p.proj_pr, p.proj_rev) == (16, 15, 111)
     assert (p.reset_iter, p.table_size, p.rounds, p.entry_seed) == (
-        15, 4096, 2, 3030517)
+        15, 0, 2, 3030517)
     p = _native.make_params(100_000, 200, 100_000, 32, 100)
     assert (p.Kp, p.topk) == (7, 200)
     assert _native.check_params(p)[0] == 0
@@ -46,8 +46,8 @@ def test_params_init_mapping():
     ("dim", 0, "dim"),
     ("M", 1, "M must"),
     ("link_rev", 100, "link_rev"),
-    ("table_size", 3000, "power of two"),
     ("table_size", 2048, "exceed"),
+    ("Kp", 4, "Kp*Kd"),
     ("topk", 1000, "topk"),
     ("Kd", 17, "Kd"),
     ("reset_iter", 0, "reset_iter"),
@@ -69,7 +69,7 @@ def test_create_without_gpu_fails_cleanly():
 
 
 def test_invalid_params_rejected_before_device():
-    p = _native.make_params(1000, 64, 1000, 32, 64, table_size=1000)
+    p = _native.make_params(1000, 64, 1000, 32, 64, table_size=900)
     L = _native.lib()
     ctx = L.pg_create(0, ctypes.byref(p))
     assert not ctx

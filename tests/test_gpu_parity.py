@@ -178,6 +178,32 @@ def test_host_entry_point_matches_device_entry_point(cuda):
     np.testing.assert_array_equal(link_h, from_dev(link_d))
 
 
+def test_table_size_changes_nothing(cuda):
+    """The visited table is an exact set: the reference's 4096 slots and the
+    auto-sized table give the same search rows."""
+    import torch
+    w = workload("d200_m32_l100")
+    n = w["params"].n_base
+    base_d = to_dev(w["base"], torch)
+    rng = np.random.default_rng(3)
+    graph = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    graph_d = to_dev(graph, torch)
+    outs = []
+    for ts in (0, 4096, 1789):
+        p = _native.make_params(n, w["base"].shape[1], n, 32, 100,
+                                table_size=ts)
+        out_d = torch.empty((n, p.topk), dtype=torch.int32, device=cuda)
+        with _native.Context(p) as ctx:
+            ctx.run_search(base_d, base_d, graph_d, out_d)
+            ctx.sync()
+        outs.append(from_dev(out_d))
+    p = _native.make_params(n, w["base"].shape[1], n, 32, 100)
+    ref = cpu_ref.search(w["base"], w["base"], graph, p.Km, p.Kp, p.Kd,
+                         p.topk)
+    for o in outs:
+        np.testing.assert_array_equal(o, ref)
+
+
 def test_build_is_deterministic(cuda):
     import torch
     w = workload("d200_m32_l100")
