@@ -45,11 +45,14 @@ def profiled_traffic(kernel, workload, precision):
     return None
 
 
-def stage_bytes(kind, c, p, n, D=None, mp=None, deg=None):
-    """Algorithmic HBM bytes of one launch from its counters (FP32 rows)."""
+def stage_bytes(kind, c, p, n, D=None, mp=None, deg=None, search_row=None):
+    """Algorithmic HBM bytes of one launch from its counters.  FP32 rows for
+    prune/merge; the search reads FP32 rows, or padded BF16 rows in the
+    screened path (search_row bytes)."""
     row = 4 * p.dim
     if kind == "search":
-        return (row * (n + c["search_dist"]) + 4 * p.Kp * p.Kd *
+        srow = search_row or row
+        return (4 * p.dim * n + srow * c["search_dist"] + 4 * p.Kp * p.Kd *
                 c["search_iters"] + 4 * p.topk * n)
     if kind == "prune":
         return row * (n + c["prune_dist"]) + 4 * D * n + 4 * mp * n
@@ -59,77 +62,31 @@ def stage_bytes(kind, c, p, n, D=None, mp=None, deg=None):
     return 0
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="cfg2_1m_d200")
-    ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--precision", default="fp32", choices=["fp32"])
-    ap.add_argument("--cpu-sample", type=int, default=4000)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-recall", action="store_true")
-    ap.add_argument("--n-eval", type=int, default=10000)
-    ap.add_argument("--tune-flags", type=int, default=0,
-                    help="perf-only knobs passed to pg_params.tune_flags")
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="run a 1 GiB device copy first (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; "
-                          "--gpus must be 1"}))
-        sys.exit(2)
-
+def time_build(ctx, base_d, gt_d, proj_d, link_d, steps, warmup, label):
     import torch
-    from paragraph_amd import _native, datagen
-
-    dev = torch.device("cuda:0")
-    cfgd = datagen.CONFIGS[args.config]
-    t0 = time.time()
-    w = datagen.make_config(args.config, scale=args.scale,
-                            n_eval=0 if args.no_recall else args.n_eval,
-                            device="cuda")
-    n, dim = w["base"].shape
-    M, L = w["M"], w["L"]
-    log(f"workload {args.config} n={n} dim={dim} M={M} L={L} generated in "
-        f"{time.time() - t0:.1f}s")
-    gt_u32 = np.ascontiguousarray(w["gt"].view(np.uint32))
-    p = _native.make_params(n, dim, n, M, L, tune_flags=args.tune_flags)
-    base_d = torch.from_numpy(w["base"]).to(dev)
-    gt_d = torch.from_numpy(w["gt"]).to(dev)
-    proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
-    link_d = torch.empty((n, M), dtype=torch.int32, device=dev)
-    ctx = _native.Context(p)
-    if args.calib_copy:
-        # a plain 1 GiB device copy, the reference point for HBM counters
-        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    for i in range(args.warmup):
+    for i in range(warmup):
         ctx.stage_device(base_d, gt_d, proj_d, link_d)
         ctx.wait()
-        log(f"warmup {i} done")
+        log(f"{label} warmup {i} done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         ctx.stage_device(base_d, gt_d, proj_d, link_d)
         ctx.wait()
-        log(f"step {i} done")
+        log(f"{label} step {i} done")
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = n / (ms_per_step / 1000.0)
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, "
-        f"{value:.0f} pts/s")
+    return 1000.0 * (time.perf_counter() - t0) / steps
 
+
+def roofline_of(ctx, p, n, workload, precision):
+    """Dominant kernel of the last build: algorithmic bytes per launch over
+    its mean launch time (HIP events on the build stream)."""
+    from paragraph_amd import _native
     st = ctx.stats()
     stage_ms = [float(st.stage_ms[i]) for i in range(st.n_stages)]
     names = [_native.stage_name(i) for i in range(st.n_stages)]
     counters = [ctx.stage_counters(i) for i in range(st.n_stages)]
+    row = 2 * ((p.dim + 31) // 32 * 32) if precision == "bf16" else 4 * p.dim
     kinds = {}
     for i, nm in enumerate(names):
         kind = nm.split("_")[0]
@@ -138,7 +95,8 @@ def main():
         D = p.gt_k if nm == "prune_proj" else p.topk
         mp = p.proj_pr if nm == "prune_proj" else p.M
         deg = p.proj_deg if nm == "merge_proj" else p.M
-        b = stage_bytes(kind, counters[i], p, n, D=D, mp=mp, deg=deg)
+        b = stage_bytes(kind, counters[i], p, n, D=D, mp=mp, deg=deg,
+                        search_row=row)
         k = kinds.setdefault(kind, {"ms": 0.0, "bytes": 0, "launches": 0})
         k["ms"] += stage_ms[i]
         k["bytes"] += b
@@ -153,72 +111,178 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(kname, args.config,
-                                            args.precision),
+                "traffic": profiled_traffic(kname, workload, precision),
                 "kernel": kname,
                 "avg_launch_ms": round(avg_ms, 3),
                 "alg_bytes_per_launch": int(bytes_per_launch)}
-    log("stage ms:", {nm: round(x, 2) for nm, x in zip(names, stage_ms)})
+    detail = {"stage_ms": {nm: round(x, 3) for nm, x in zip(names, stage_ms)},
+              "stage_counters": {nm: c for nm, c in zip(names, counters)}}
+    return roofline, detail
 
-    extra = {"stage_ms": {nm: round(x, 3) for nm, x in zip(names, stage_ms)},
-             "stage_counters": {nm: c for nm, c in zip(names, counters)},
-             "counters": st.as_dict()}
-    extra["counters"].pop("stage_ms", None)
 
-    # ---- recall@10 of the link graph on held-out queries (GPU search)
+def recall_at_10(eval_ctx, eq_d, base_d, link_d, truth, topk):
+    import torch
+    out_d = torch.empty((eq_d.shape[0], topk), dtype=torch.int32,
+                        device=eq_d.device)
+    eval_ctx.run_search(eq_d, base_d, link_d, out_d)
+    eval_ctx.sync()
+    found = out_d[:, :10].cpu().numpy()
+    hits = sum(len(set(a) & set(b)) for a, b in zip(found, truth))
+    return hits / truth.size
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg2_1m_d200")
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--precision", default="both",
+                    choices=["fp32", "bf16", "both"],
+                    help="both: time the plain FP32 path and the BF16-"
+                         "screened path; value = the screened path")
+    ap.add_argument("--cpu-sample", type=int, default=2000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-recall", action="store_true")
+    ap.add_argument("--n-eval", type=int, default=10000)
+    ap.add_argument("--tune-flags", type=int, default=0,
+                    help="perf-only knobs for the FP32 build")
+    ap.add_argument("--tune-flags-bf16", type=int, default=0,
+                    help="perf-only knobs for the BF16 build")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run a 1 GiB device copy first (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; "
+                          "--gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    from paragraph_amd import _native, datagen
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    w = datagen.make_config(args.config, scale=args.scale,
+                            n_eval=0 if args.no_recall else args.n_eval,
+                            device="cuda")
+    n, dim = w["base"].shape
+    M, L = w["M"], w["L"]
+    log(f"workload {args.config} n={n} dim={dim} M={M} L={L} generated in "
+        f"{time.time() - t0:.1f}s")
+    gt_u32 = np.ascontiguousarray(w["gt"].view(np.uint32))
+    base_d = torch.from_numpy(w["base"]).to(dev)
+    gt_d = torch.from_numpy(w["gt"]).to(dev)
+    if args.calib_copy:
+        # a plain 1 GiB device copy, the reference point for HBM counters
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    precs = ["fp32", "bf16"] if args.precision == "both" else [args.precision]
+    results = {}
+    p32 = _native.make_params(n, dim, n, M, L, tune_flags=args.tune_flags)
+    eval_ctx = None
+    truth = None
     if not args.no_recall:
         eq = w["eval_queries"]
         eq_d = torch.from_numpy(eq).to(dev)
-        out_d = torch.empty((eq.shape[0], p.topk), dtype=torch.int32,
-                            device=dev)
-        ctx.run_search(eq_d, base_d, link_d, out_d)
-        ctx.sync()
-        found = out_d[:, :10].cpu().numpy()
-        # exact top-10 in float64
         B64 = base_d.double()
-        truth = []
+        tl = []
         for s in range(0, eq.shape[0], 1000):
             sc = eq_d[s:s + 1000].double() @ B64.T
-            truth.append(torch.topk(sc, 10, dim=1).indices.cpu().numpy())
-        truth = np.concatenate(truth)
-        hits = sum(len(set(a) & set(b)) for a, b in zip(found, truth))
-        extra["recall_at_10_link_graph"] = round(hits / truth.size, 4)
-        del B64
-        log(f"recall@10 (link graph, {eq.shape[0]} held-out queries): "
-            f"{extra['recall_at_10_link_graph']}")
+            tl.append(torch.topk(sc, 10, dim=1).indices.cpu().numpy())
+        truth = np.concatenate(tl)
+        del B64, sc
+        torch.cuda.empty_cache()
+    for prec in precs:
+        if prec == "fp32":
+            p = p32
+        else:
+            p = _native.make_params(n, dim, n, M, L,
+                                    precision=_native.PG_BF16,
+                                    tune_flags=args.tune_flags_bf16)
+        proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
+        link_d = torch.empty((n, M), dtype=torch.int32, device=dev)
+        ctx = _native.Context(p)
+        ms = time_build(ctx, base_d, gt_d, proj_d, link_d, args.steps,
+                        args.warmup, prec)
+        roof, detail = roofline_of(ctx, p, n, args.config, prec)
+        r = {"ms_per_step": ms, "value": n / (ms / 1000.0),
+             "roofline": roof, **detail,
+             "counters": {k: v for k, v in ctx.stats().as_dict().items()
+                          if k != "stage_ms"}}
+        log(f"{prec}: {ms:.1f} ms/step, {r['value']:.0f} pts/s; stage ms "
+            f"{ {k: round(v, 1) for k, v in detail['stage_ms'].items()} }")
+        if truth is not None:
+            if eval_ctx is None:
+                eval_ctx = _native.Context(p32)
+            r["recall_at_10"] = round(recall_at_10(
+                eval_ctx, eq_d, base_d, link_d, truth, p32.topk), 4)
+            log(f"{prec}: recall@10 of the link graph "
+                f"({truth.shape[0]} held-out queries) = {r['recall_at_10']}")
+        r["link"] = link_d
+        r["ctx"] = ctx
+        results[prec] = r
 
     # ---- CPU baseline: the oracle on a bounded node sample
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(ctx, p, w, gt_u32, base_d, gt_d, args.cpu_sample,
-                           log)
+        ctx32 = results["fp32"]["ctx"] if "fp32" in results else \
+            _native.Context(p32)
+        cpu = cpu_baseline(ctx32, p32, w, gt_u32, base_d, gt_d,
+                           args.cpu_sample, log)
 
+    head = results["bf16"] if "bf16" in results else results["fp32"]
+    head_prec = "bf16" if "bf16" in results else "fp32"
+    extra = {}
+    for prec, r in results.items():
+        extra[prec] = {k: (round(v, 3) if isinstance(v, float) else v)
+                       for k, v in r.items() if k not in ("link", "ctx")}
+    if "fp32" in results and "bf16" in results:
+        extra["speedup_bf16_vs_fp32"] = round(
+            results["fp32"]["ms_per_step"] / results["bf16"]["ms_per_step"], 3)
+        if truth is not None:
+            extra["recall_delta_pt"] = round(
+                100 * (results["bf16"]["recall_at_10"] -
+                       results["fp32"]["recall_at_10"]), 2)
+        a = results["fp32"]["link"].cpu().numpy()
+        b = results["bf16"]["link"].cpu().numpy()
+        extra["link_overlap_bf16_vs_fp32"] = round(float(np.mean([
+            len(set(x[x >= 0]) & set(y[y >= 0])) / max(1, len(set(x[x >= 0])))
+            for x, y in zip(a[:20000], b[:20000])])), 4)
     line = {
         "metric": "base points indexed/sec, 1 MI355X, FP32 vs BF16-MFMA at "
                   "recall@10 within 0.5 pt",
-        "value": round(value, 1),
+        "value": round(head["value"], 1),
         "unit": "base points/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2),
+        "ms_per_step": round(head["ms_per_step"], 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": head_prec,
         "data": "synthetic (seeded 64-component Gaussian mixture; exact "
                 "top-128 inner-product gt)",
         "config": {"workload": f"{args.config}: {n} base x {dim} f32, {n} "
                    f"training queries, gt k=128, M={M}, L={L}, inner "
-                   "product, precision fp32",
+                   f"product; value = {head_prec} path"
+                   + (" (BF16 MFMA search screening, FP32 re-rank and "
+                      "prune)" if head_prec == "bf16" else ""),
                    "n_base": n, "dim": dim, "M": M, "L": L,
-                   "precision": args.precision},
-        "roofline": roofline,
+                   "precision": head_prec},
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)
-    ctx.close()
+    for r in results.values():
+        r["ctx"].close()
 
 
 def cpu_baseline(ctx, p, w, gt_u32, base_d, gt_d, sample, log):

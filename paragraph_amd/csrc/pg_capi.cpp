@@ -49,6 +49,8 @@ struct pg_ctx {
   uint32_t* scan_tmp = nullptr;
   uint32_t* srcs = nullptr;
   uint32_t* res = nullptr;
+  void* base16 = nullptr;   // screened paths: BF16 copy of the base
+  uint32_t dpad = 0;        // its row stride (dim rounded up to 32)
   unsigned long long* stats = nullptr;
   unsigned long long* stage_cnt = nullptr;  // (kMaxStages+1) x 16 snapshots
   // staging for the host-pointer entry point (allocated on first use)
@@ -140,6 +142,8 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   hipStream_t s = c->stream;
   int ev = 0;
   PG_TRY(record(c, ev++));
+  if (c->base16)  // screened path: BF16 copy of the base for the search
+    PG_HIP(c, pg::launch_to_bf16(d_base, N, p.dim, c->dpad, c->base16, s));
   PG_HIP(c, pg::launch_match(d_gt, p.n_query, p.gt_k, N, c->winner, s));
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_prune(d_base, N, p.dim, nullptr, p.gt_k, d_gt,
@@ -156,7 +160,8 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   const uint32_t* graph = d_proj;
   uint32_t gdeg = p.proj_deg;
   for (uint32_t r = 0; r < p.rounds; ++r) {
-    PG_HIP(c, pg::launch_search(d_base, N, d_base, N, p.dim, graph, gdeg, p.Km,
+    PG_HIP(c, pg::launch_search(d_base, N, d_base, c->base16, c->dpad, N,
+                                p.dim, graph, gdeg, p.Km,
                                 p.Kp, p.Kd, p.topk, p.reset_iter,
                                 p.entry_seed, p.max_iters,
                                 effective_table_size(p), tune_gather(p),
@@ -254,8 +259,8 @@ int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len) {
   }
   if (p->rounds > 3) return fail("rounds must be <= 3");
   if (p->max_iters == 0) return fail("max_iters must be > 0");
-  if (p->precision != PG_FP32)
-    return fail("only the FP32 precision path is available in this build");
+  if (p->precision != PG_FP32 && p->precision != PG_BF16)
+    return fail("precision must be PG_FP32 or PG_BF16 in this build");
   if (msg && msg_len) msg[0] = 0;
   return PG_OK;
 }
@@ -284,6 +289,7 @@ pg_ctx* pg_create(int device, const pg_params* p) {
   c->device = device;
   c->p = *p;
   c->mp_max = p->proj_pr > p->M ? p->proj_pr : p->M;
+  c->dpad = (p->dim + 31) / 32 * 32;
   auto fail = [&](int) {
     std::memcpy(g_err, c->err, sizeof(g_err));
     pg_destroy(c);
@@ -313,6 +319,9 @@ pg_ctx* pg_create(int device, const pg_params* p) {
       dev_alloc(c, &c->scan_tmp, nb, "scan_tmp") ||
       dev_alloc(c, &c->srcs, N * c->mp_max, "srcs") ||
       dev_alloc(c, &c->res, N * p->topk, "search_out") ||
+      (p->precision == PG_BF16 &&
+       dev_alloc(c, reinterpret_cast<uint16_t**>(&c->base16),
+                 N * ((p->dim + 31) / 32 * 32), "base_bf16")) ||
       dev_alloc(c, &c->stats, kStatCount, "stats") ||
       dev_alloc(c, &c->stage_cnt, (kMaxStages + 1) * 16ull, "stage_cnt"))
     return fail(0);
@@ -337,6 +346,8 @@ void pg_destroy(pg_ctx* c) {
   dev_free(c->scan_tmp);
   dev_free(c->srcs);
   dev_free(c->res);
+  if (c->base16) (void)hipFree(c->base16);
+  c->base16 = nullptr;
   dev_free(c->stats);
   dev_free(c->stage_cnt);
   dev_free(c->h_base_dev);
@@ -525,7 +536,11 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
     return set_err(c, PG_ERR_INVALID, "Kd (%u) exceeds graph degree (%u)", p.Kd,
                    graph_deg);
   PG_HIP(c, hipSetDevice(c->device));
-  PG_HIP(c, pg::launch_search(d_queries, n_query, d_base, p.n_base, p.dim,
+  if (c->base16)
+    PG_HIP(c, pg::launch_to_bf16(d_base, p.n_base, p.dim, c->dpad, c->base16,
+                                 c->stream));
+  PG_HIP(c, pg::launch_search(d_queries, n_query, d_base, c->base16, c->dpad,
+                              p.n_base, p.dim,
                               d_graph, graph_deg, p.Km, p.Kp, p.Kd, p.topk,
                               p.reset_iter, p.entry_seed, p.max_iters,
                               effective_table_size(p), tune_gather(p), d_out,

@@ -30,8 +30,13 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
                         hipStream_t s);
 size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
                          uint32_t Kp);
+hipError_t launch_to_bf16(const float* base, uint32_t n, uint32_t dim,
+                          uint32_t dpad, void* out, hipStream_t s);
+// base16 == nullptr: exact FP32 search; else BF16-screened search over the
+// padded BF16 copy (row stride dpad elements).
 hipError_t launch_search(const float* queries, uint32_t n_query,
-                         const float* base, uint32_t n_base, uint32_t dim,
+                         const float* base, const void* base16, uint32_t dpad,
+                         uint32_t n_base, uint32_t dim,
                          const uint32_t* graph, uint32_t gdeg, uint32_t Km,
                          uint32_t Kp, uint32_t Kd, uint32_t topk,
                          uint32_t reset_iter, uint32_t entry_seed,

@@ -450,45 +450,146 @@ __device__ __forceinline__ bool visit_insert(uint32_t* tab, uint32_t tsize,
   }
 }
 
+// Distance evaluators for the search.  Each one computes the distances of
+// the todo-listed candidates to the query and writes their list keys.
+//
+// EvalF32: the reference's exact order.  Half a wave (32 lanes) per
+// candidate; lane l holds query elements l, l+32, ... in registers and runs
+// one FMA chain over them, then the xor tree 16/8/4/2/1 (lane 0 of each half
+// ends with exactly the reference's __shfl_down_sync result).  U candidate
+// pairs have their rows in flight together.
 template <int TQ, int U>
-__device__ __forceinline__ void eval_misses(
-    const float (&qv)[TQ], const float* __restrict__ base, uint32_t dim,
-    const uint32_t* cand, const uint32_t* todo, uint32_t ntodo, uint64_t* tail) {
-  const uint32_t lane = lane_id();
-  const uint32_t half = lane >> 5, l = lane & 31u;
-  for (uint32_t p0 = 0; p0 < ntodo; p0 += 2 * U) {
-    uint32_t tt[U], cid[U];
-    bool ok[U];
-    float xv[U][TQ];
+struct EvalF32 {
+  float qv[TQ];
+  const float* __restrict__ base;
+  uint32_t dim;
+
+  __device__ __forceinline__ void load_query(const float* __restrict__ queries,
+                                             uint32_t q) {
+    const uint32_t l = lane_id() & 31u;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t idx = p0 + 2 * u + half;
-      ok[u] = idx < ntodo;
-      tt[u] = ok[u] ? todo[idx] : 0u;
-      cid[u] = ok[u] ? cand[tt[u]] : 0u;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const float* row = base + size_t(cid[u]) * dim;
-#pragma unroll
-      for (int t = 0; t < TQ; ++t) {
-        const uint32_t e = l + 32u * t;
-        xv[u][t] = (ok[u] && e < dim) ? row[e] : 0.0f;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      float s = 0.0f;
-#pragma unroll
-      for (int t = 0; t < TQ; ++t) {
-        if (l + 32u * t < dim) s = __builtin_fmaf(qv[t], xv[u][t], s);
-      }
-#pragma unroll
-      for (int o = 16; o > 0; o >>= 1) s = s + __shfl_xor(s, o, 64);
-      if (l == 0 && ok[u]) tail[tt[u]] = search_key(-s, cid[u]);
+    for (int t = 0; t < TQ; ++t) {
+      const uint32_t e = l + 32u * t;
+      qv[t] = e < dim ? queries[size_t(q) * dim + e] : 0.0f;
     }
   }
-}
+
+  __device__ __forceinline__ void eval(const uint32_t* cand,
+                                       const uint32_t* todo, uint32_t ntodo,
+                                       uint64_t* tail) const {
+    const uint32_t lane = lane_id();
+    const uint32_t half = lane >> 5, l = lane & 31u;
+    for (uint32_t p0 = 0; p0 < ntodo; p0 += 2 * U) {
+      uint32_t tt[U], cid[U];
+      bool ok[U];
+      float xv[U][TQ];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t idx = p0 + 2 * u + half;
+        ok[u] = idx < ntodo;
+        tt[u] = ok[u] ? todo[idx] : 0u;
+        cid[u] = ok[u] ? cand[tt[u]] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const float* row = base + size_t(cid[u]) * dim;
+#pragma unroll
+        for (int t = 0; t < TQ; ++t) {
+          const uint32_t e = l + 32u * t;
+          xv[u][t] = (ok[u] && e < dim) ? row[e] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        float s = 0.0f;
+#pragma unroll
+        for (int t = 0; t < TQ; ++t) {
+          if (l + 32u * t < dim) s = __builtin_fmaf(qv[t], xv[u][t], s);
+        }
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) s = s + __shfl_xor(s, o, 64);
+        if (l == 0 && ok[u]) tail[tt[u]] = search_key(-s, cid[u]);
+      }
+    }
+  }
+};
+
+// EvalBF16: screening distances on MFMA.  The base is a BF16 copy with rows
+// zero-padded to dpad (a multiple of 32).  16 candidates form the A operand
+// of v_mfma_f32_16x16x32_bf16 (lane l holds row l%16, elements
+// 32c + 8(l/16) .. +7 of chunk c); the query, broadcast to all 16 columns,
+// is the B operand; FP32 accumulation over NCH chunks.  Lane l of the result
+// holds rows 4(l/16) .. +3.  G groups of 16 are in flight together.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int NCH, int G>
+struct EvalBF16 {
+  bf16x8 qf[NCH];
+  const __bf16* __restrict__ base16;
+  uint32_t dim, dpad, nch;
+
+  __device__ __forceinline__ void load_query(const float* __restrict__ queries,
+                                             uint32_t q) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t e = 32u * c + 8u * (lane >> 4) + j;
+        qf[c][j] = (__bf16)((uint32_t(c) < nch && e < dim)
+                                ? queries[size_t(q) * dim + e]
+                                : 0.0f);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void eval(const uint32_t* cand,
+                                       const uint32_t* todo, uint32_t ntodo,
+                                       uint64_t* tail) const {
+    const uint32_t lane = lane_id();
+    const uint32_t m = lane & 15u, kq = lane >> 4;
+    for (uint32_t g0 = 0; g0 < ntodo; g0 += 16u * G) {
+      bf16x8 a[G][NCH];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const uint32_t idx = g0 + 16u * g + m;
+        const bool ok = idx < ntodo;
+        const uint32_t cid = ok ? cand[todo[idx]] : 0u;
+        const bf16x8* row = reinterpret_cast<const bf16x8*>(
+            base16 + size_t(cid) * dpad + 8u * kq);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          if (ok && uint32_t(c) < nch) {
+            a[g][c] = row[4 * c];
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) a[g][c][j] = (__bf16)0.0f;
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g][c], qf[c], acc,
+                                                        0, 0, 0);
+        // lane l (l%16 < 4) writes candidate 4(l/16) + l%4
+        if (m < 4u) {
+          const uint32_t cm = 4u * kq + m;
+          const uint32_t idx = g0 + 16u * g + cm;
+          const float dot = m == 0 ? acc[0] : m == 1 ? acc[1]
+                          : m == 2 ? acc[2] : acc[3];
+          if (idx < ntodo) {
+            const uint32_t t = todo[idx];
+            tail[t] = search_key(-dot, cand[t]);
+          }
+        }
+      }
+    }
+  }
+};
 
 // Merges the sorted misses of one iteration into the top-Km list (the
 // reference re-sorts the whole list, Gsearcher.cuh:1045; only its first Km
@@ -551,10 +652,10 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
   return false;
 }
 
-template <int TQ, int U>
+template <class Eval>
 __global__ __launch_bounds__(64) void search_kernel(
-    const float* __restrict__ queries, uint32_t n_query,
-    const float* __restrict__ base, uint32_t n_base, uint32_t dim,
+    Eval ev, const float* __restrict__ queries, uint32_t n_query,
+    uint32_t n_base,
     const uint32_t* __restrict__ graph, uint32_t gdeg, uint32_t Km,
     uint32_t Kp, uint32_t Kd, uint32_t topk, uint32_t reset_iter,
     uint32_t entry_seed, uint32_t max_iters, uint32_t tsize, uint32_t lp2,
@@ -570,14 +671,8 @@ __global__ __launch_bounds__(64) void search_kernel(
   uint32_t* explore = todo + kpkd;
   const uint32_t q = blockIdx.x;
   const uint32_t lane = lane_id();
-  const uint32_t l = lane & 31u;
   const bool fast_merge = Km <= 256 && kpkd <= 256;
-  float qv[TQ];
-#pragma unroll
-  for (int t = 0; t < TQ; ++t) {
-    const uint32_t e = l + 32u * t;
-    qv[t] = e < dim ? queries[size_t(q) * dim + e] : 0.0f;
-  }
+  ev.load_query(queries, q);
   // initial list: [0, Km) = (FLT_MAX, tomb); [Km, Km+kpkd) = entry points
   for (uint32_t i = lane; i < lp2; i += kWave)
     keys[i] = i < Km ? search_key(FLT_MAX, kTomb) : kPadKey;
@@ -587,7 +682,7 @@ __global__ __launch_bounds__(64) void search_kernel(
   }
   __syncthreads();
   uint64_t* tail = keys + Km;
-  eval_misses<TQ, U>(qv, base, dim, cand, todo, kpkd, tail);
+  ev.eval(cand, todo, kpkd, tail);
   unsigned long long ndist = kpkd;
   __syncthreads();
   bitonic_sort(keys, lp2);
@@ -650,7 +745,7 @@ __global__ __launch_bounds__(64) void search_kernel(
       ntodo += __popcll(m);
     }
     __syncthreads();
-    eval_misses<TQ, U>(qv, base, dim, cand, todo, ntodo, tail);
+    ev.eval(cand, todo, ntodo, tail);
     ndist += ntodo;
     __syncthreads();
     const bool last_allowed = total_iter + 1 >= max_iters;
@@ -776,8 +871,64 @@ size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
          size_t(kpkd) * 16 + size_t(Kp) * 4 + 16;
 }
 
+template <class Eval>
+static hipError_t launch_search_eval(
+    const Eval& ev, const float* queries, uint32_t n_query, uint32_t n_base,
+    const uint32_t* graph, uint32_t gdeg, uint32_t Km, uint32_t Kp,
+    uint32_t Kd, uint32_t topk, uint32_t reset_iter, uint32_t entry_seed,
+    uint32_t max_iters, uint32_t table_size, uint32_t* out,
+    unsigned long long* stats, hipStream_t s) {
+  uint32_t lp2 = 64;
+  while (lp2 < Km + Kp * Kd) lp2 <<= 1;
+  const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&search_kernel<Eval>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(smem));
+  search_kernel<Eval><<<n_query, 64, smem, s>>>(
+      ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,
+      entry_seed, max_iters, table_size, lp2, out, stats);
+  return hipGetLastError();
+}
+
+template <int TQ, int U>
+static hipError_t launch_f32(const float* base, uint32_t dim,
+                             const float* queries, uint32_t n_query,
+                             uint32_t n_base, const uint32_t* graph,
+                             uint32_t gdeg, uint32_t Km, uint32_t Kp,
+                             uint32_t Kd, uint32_t topk, uint32_t reset_iter,
+                             uint32_t entry_seed, uint32_t max_iters,
+                             uint32_t table_size, uint32_t* out,
+                             unsigned long long* stats, hipStream_t s) {
+  EvalF32<TQ, U> ev;
+  ev.base = base;
+  ev.dim = dim;
+  return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
+                            Kd, topk, reset_iter, entry_seed, max_iters,
+                            table_size, out, stats, s);
+}
+
+template <int NCH, int G>
+static hipError_t launch_bf16(const void* base16, uint32_t dim, uint32_t dpad,
+                              const float* queries, uint32_t n_query,
+                              uint32_t n_base, const uint32_t* graph,
+                              uint32_t gdeg, uint32_t Km, uint32_t Kp,
+                              uint32_t Kd, uint32_t topk, uint32_t reset_iter,
+                              uint32_t entry_seed, uint32_t max_iters,
+                              uint32_t table_size, uint32_t* out,
+                              unsigned long long* stats, hipStream_t s) {
+  EvalBF16<NCH, G> ev;
+  ev.base16 = reinterpret_cast<const __bf16*>(base16);
+  ev.dim = dim;
+  ev.dpad = dpad;
+  ev.nch = dpad / 32;
+  return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
+                            Kd, topk, reset_iter, entry_seed, max_iters,
+                            table_size, out, stats, s);
+}
+
 hipError_t launch_search(const float* queries, uint32_t n_query,
-                         const float* base, uint32_t n_base, uint32_t dim,
+                         const float* base, const void* base16, uint32_t dpad,
+                         uint32_t n_base, uint32_t dim,
                          const uint32_t* graph, uint32_t gdeg, uint32_t Km,
                          uint32_t Kp, uint32_t Kd, uint32_t topk,
                          uint32_t reset_iter, uint32_t entry_seed,
@@ -785,36 +936,63 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t gather_batch, uint32_t* out,
                          unsigned long long* stats, hipStream_t s) {
   if (n_query == 0) return hipSuccess;
-  uint32_t lp2 = 64;
-  while (lp2 < Km + Kp * Kd) lp2 <<= 1;
-  const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp);
+#define PG_ARGS                                                             \
+  queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,      \
+      entry_seed, max_iters, table_size, out, stats, s
+  if (base16 != nullptr) {
+    // screened path: gather_batch = groups of 16 candidates in flight
+    const uint32_t nch = dpad / 32;
+    const bool two = gather_batch >= 2;
+    if (nch <= 8) {
+      return two ? launch_bf16<8, 2>(base16, dim, dpad, PG_ARGS)
+                 : launch_bf16<8, 1>(base16, dim, dpad, PG_ARGS);
+    }
+    if (nch <= 16) {
+      return two ? launch_bf16<16, 2>(base16, dim, dpad, PG_ARGS)
+                 : launch_bf16<16, 1>(base16, dim, dpad, PG_ARGS);
+    }
+    if (nch <= 32) return launch_bf16<32, 1>(base16, dim, dpad, PG_ARGS);
+    return hipErrorInvalidValue;
+  }
   const uint32_t tq = (dim + 31) / 32;
   // gather_batch: candidate pairs whose rows are in flight together
-  // (perf only); 0 picks by the query slice size.
-  uint32_t u = gather_batch;
-  if (u == 0) u = 4;
-#define PG_SEARCH(T, UU)                                                      \
-  do {                                                                        \
-    (void)hipFuncSetAttribute(                                                \
-        reinterpret_cast<const void*>(&search_kernel<T, UU>),                 \
-        hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));               \
-    search_kernel<T, UU><<<n_query, 64, smem, s>>>(                           \
-        queries, n_query, base, n_base, dim, graph, gdeg, Km, Kp, Kd, topk,   \
-        reset_iter, entry_seed, max_iters, table_size, lp2, out, stats);      \
+  // (perf only); 0 = 4.
+  const uint32_t u = gather_batch == 0 ? 4u : gather_batch;
+#define PG_F32_U(T)                                                       \
+  do {                                                                    \
+    if (u <= 4) return launch_f32<T, 4>(base, dim, PG_ARGS);              \
+    if (u <= 8) return launch_f32<T, 8>(base, dim, PG_ARGS);              \
+    return launch_f32<T, 16>(base, dim, PG_ARGS);                         \
   } while (0)
-#define PG_SEARCH_U(T)                 \
-  do {                                 \
-    if (u <= 4) PG_SEARCH(T, 4);       \
-    else if (u <= 8) PG_SEARCH(T, 8);  \
-    else PG_SEARCH(T, 16);             \
-  } while (0)
-  if (tq <= 4) PG_SEARCH_U(4);
-  else if (tq <= 8) PG_SEARCH_U(8);
-  else if (tq <= 16) PG_SEARCH_U(16);
-  else if (tq <= 32) PG_SEARCH(32, 4);
-  else return hipErrorInvalidValue;
-#undef PG_SEARCH_U
-#undef PG_SEARCH
+  if (tq <= 4) PG_F32_U(4);
+  if (tq <= 8) PG_F32_U(8);
+  if (tq <= 16) PG_F32_U(16);
+  if (tq <= 32) return launch_f32<32, 4>(base, dim, PG_ARGS);
+#undef PG_F32_U
+#undef PG_ARGS
+  return hipErrorInvalidValue;
+}
+
+// Row-major FP32 base -> BF16 copy with rows zero-padded to dpad (round to
+// nearest even), one thread per output element pair.
+__global__ void to_bf16_kernel(const float* __restrict__ base, uint64_t n,
+                               uint32_t dim, uint32_t dpad,
+                               __bf16* __restrict__ out) {
+  const uint64_t total = n * dpad;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < total;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t r = i / dpad;
+    const uint32_t c = uint32_t(i - r * dpad);
+    out[i] = (__bf16)(c < dim ? base[r * dim + c] : 0.0f);
+  }
+}
+
+hipError_t launch_to_bf16(const float* base, uint32_t n, uint32_t dim,
+                          uint32_t dpad, void* out, hipStream_t s) {
+  const uint64_t total = uint64_t(n) * dpad;
+  if (total == 0) return hipSuccess;
+  to_bf16_kernel<<<grid_for(total, 256), 256, 0, s>>>(
+      base, n, dim, dpad, reinterpret_cast<__bf16*>(out));
   return hipGetLastError();
 }
 
