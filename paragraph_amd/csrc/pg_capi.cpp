@@ -9,6 +9,7 @@
 // separate, typed allocations here.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -50,7 +51,9 @@ struct pg_ctx {
   uint32_t* srcs = nullptr;
   uint32_t* res = nullptr;
   void* base16 = nullptr;   // screened paths: BF16 copy of the base
-  uint32_t dpad = 0;        // its row stride (dim rounded up to 32)
+  float* norms = nullptr;   // screened paths: FP32 row norms
+  uint32_t dpad = 0;        // BF16 row stride (dim rounded up to 32)
+  float eps_c = 0.0f;       // screening bound factor
   unsigned long long* stats = nullptr;
   unsigned long long* stage_cnt = nullptr;  // (kMaxStages+1) x 16 snapshots
   // staging for the host-pointer entry point (allocated on first use)
@@ -102,8 +105,14 @@ void dev_free(T*& p) {
 
 // Visited-table slots actually used: table_size, or (0 = auto) the most ids
 // one reset period can insert plus 1/16 slack, rounded up to 64 slots.
+// The screened search uses the table as a visited cache (see the kernel); its
+// auto size is 4*(Km + Kp*Kd) rounded up to 64 slots.
 uint32_t effective_table_size(const pg_params& p) {
   if (p.table_size) return p.table_size;
+  if (p.precision != PG_FP32) {
+    const uint64_t t = (4ull * (p.Km + uint64_t(p.Kp) * p.Kd) + 63) / 64 * 64;
+    return uint32_t(t);
+  }
   const uint64_t bound = uint64_t(p.Km) + uint64_t(p.reset_iter) * p.Kp * p.Kd;
   uint64_t t = bound + (bound / 16 > 64 ? bound / 16 : 64);
   t = (t + 63) / 64 * 64;
@@ -113,6 +122,24 @@ uint32_t effective_table_size(const pg_params& p) {
 // Perf-only knobs packed in tune_flags (0 = library default):
 //   bits 0-4  search gather batch (candidate pairs in flight; 4, 8 or 16)
 uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
+
+// Bound factor of the BF16 screen against the exact FP32 chain:
+// |screened - chain| <= eps * sum|x_i y_i| <= eps * |x| |y| with
+//   BF16 rounding of both operands (2u + u^2, u = 2^-8; the products of
+//   BF16 values are exact in FP32),
+//   FP32 accumulation inside the MFMA over dpad terms (gamma taken with
+//   2^-23 to allow any rounding mode; the terms sum to at most
+//   (1 + u)^2 sum|x y|),
+//   the chain's own FP32 error (gamma over dim terms, u = 2^-24),
+// inflated by 1% for the FP32 norms and the comparisons themselves.
+float screen_eps(uint32_t dim, uint32_t dpad) {
+  const double u = 1.0 / 256.0;
+  const double ua = std::ldexp(1.0, -23);
+  const double ga = dpad * ua / (1.0 - dpad * ua);
+  const double gc = dim * std::ldexp(1.0, -24) / (1.0 - dim * std::ldexp(1.0, -24));
+  const double e = (2 * u + u * u) + ga * (1 + u) * (1 + u) + gc;
+  return float(e * 1.01);
+}
 
 uint32_t next_pow2(uint32_t x) {
   uint32_t v = 1;
@@ -135,6 +162,17 @@ int record(pg_ctx* c, int idx) {
     if (rc_ != PG_OK) return rc_; \
   } while (0)
 
+pg::ScreenArgs screen_args(const pg_ctx* c) {
+  return pg::ScreenArgs{reinterpret_cast<const __bf16*>(c->base16), c->norms,
+                        c->dpad, c->eps_c};
+}
+
+int prepare_screen(pg_ctx* c, const float* d_base) {
+  PG_HIP(c, pg::launch_prepare_screen(d_base, c->p.n_base, c->p.dim, c->dpad,
+                                      c->base16, c->norms, c->stream));
+  return PG_OK;
+}
+
 int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
                  uint32_t* d_proj, uint32_t* d_link) {
   const pg_params& p = c->p;
@@ -142,20 +180,22 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   hipStream_t s = c->stream;
   int ev = 0;
   PG_TRY(record(c, ev++));
-  if (c->base16)  // screened path: BF16 copy of the base for the search
-    PG_HIP(c, pg::launch_to_bf16(d_base, N, p.dim, c->dpad, c->base16, s));
+  const pg::ScreenArgs sc = screen_args(c);
+  const pg::ScreenArgs* scp = c->base16 ? &sc : nullptr;
+  if (c->base16)  // screened path: BF16 split of the base and row norms
+    PG_TRY(prepare_screen(c, d_base));
   PG_HIP(c, pg::launch_match(d_gt, p.n_query, p.gt_k, N, c->winner, s));
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_prune(d_base, N, p.dim, nullptr, p.gt_k, d_gt,
                              c->winner, p.gt_k, p.proj_pr, c->pruned, c->pcnt,
-                             c->stats, s));
+                             c->stats, nullptr, s));
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.proj_pr, c->deg,
                                c->off, c->cursor, c->scan_tmp, c->srcs, s));
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_merge(d_base, N, p.dim, c->off, c->srcs, p.proj_rev,
                              c->pruned, c->pcnt, p.proj_pr, p.proj_deg, 0,
-                             d_proj, c->stats, s));
+                             d_proj, c->stats, nullptr, s));
   PG_TRY(record(c, ev++));
   const uint32_t* graph = d_proj;
   uint32_t gdeg = p.proj_deg;
@@ -169,14 +209,14 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, p.topk, nullptr,
                                nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
-                               s));
+                               scp, s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.M, c->deg, c->off,
                                  c->cursor, c->scan_tmp, c->srcs, s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_merge(d_base, N, p.dim, c->off, c->srcs, p.link_rev,
                                c->pruned, c->pcnt, p.M, p.M, 1, d_link,
-                               c->stats, s));
+                               c->stats, scp, s));
     PG_TRY(record(c, ev++));
     graph = d_link;
     gdeg = p.M;
@@ -245,8 +285,12 @@ int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len) {
   if (p->reset_iter == 0) return fail("reset_iter must be > 0");
   if (p->Kp * p->Kd < p->Km)
     return fail("Kp*Kd must be >= Km (the entry points fill the first list)");
-  if (uint64_t(effective_table_size(*p)) <=
-      uint64_t(p->Km) + uint64_t(p->reset_iter) * p->Kp * p->Kd)
+  if (p->precision != PG_FP32) {
+    if (4ull * (p->Km + uint64_t(p->Kp) * p->Kd) >
+        3ull * effective_table_size(*p) + 3)
+      return fail("screened path: table_size must be >= 4/3 (Km + Kp*Kd)");
+  } else if (uint64_t(effective_table_size(*p)) <=
+             uint64_t(p->Km) + uint64_t(p->reset_iter) * p->Kp * p->Kd)
     return fail("table_size must exceed Km + reset_iter*Kp*Kd (else the "
                 "visited table can fill up)");
   {
@@ -261,6 +305,8 @@ int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len) {
   if (p->max_iters == 0) return fail("max_iters must be > 0");
   if (p->precision != PG_FP32 && p->precision != PG_BF16)
     return fail("precision must be PG_FP32 or PG_BF16 in this build");
+  if (p->precision == PG_BF16 && p->dim > 512)
+    return fail("the BF16-screened path supports dim <= 512");
   if (msg && msg_len) msg[0] = 0;
   return PG_OK;
 }
@@ -290,6 +336,7 @@ pg_ctx* pg_create(int device, const pg_params* p) {
   c->p = *p;
   c->mp_max = p->proj_pr > p->M ? p->proj_pr : p->M;
   c->dpad = (p->dim + 31) / 32 * 32;
+  c->eps_c = screen_eps(p->dim, c->dpad);
   auto fail = [&](int) {
     std::memcpy(g_err, c->err, sizeof(g_err));
     pg_destroy(c);
@@ -320,8 +367,9 @@ pg_ctx* pg_create(int device, const pg_params* p) {
       dev_alloc(c, &c->srcs, N * c->mp_max, "srcs") ||
       dev_alloc(c, &c->res, N * p->topk, "search_out") ||
       (p->precision == PG_BF16 &&
-       dev_alloc(c, reinterpret_cast<uint16_t**>(&c->base16),
-                 N * ((p->dim + 31) / 32 * 32), "base_bf16")) ||
+       (dev_alloc(c, reinterpret_cast<uint16_t**>(&c->base16),
+                  N * ((p->dim + 31) / 32 * 32), "base_bf16") ||
+        dev_alloc(c, &c->norms, N, "norms"))) ||
       dev_alloc(c, &c->stats, kStatCount, "stats") ||
       dev_alloc(c, &c->stage_cnt, (kMaxStages + 1) * 16ull, "stage_cnt"))
     return fail(0);
@@ -348,6 +396,7 @@ void pg_destroy(pg_ctx* c) {
   dev_free(c->res);
   if (c->base16) (void)hipFree(c->base16);
   c->base16 = nullptr;
+  dev_free(c->norms);
   dev_free(c->stats);
   dev_free(c->stage_cnt);
   dev_free(c->h_base_dev);
@@ -494,9 +543,11 @@ int pg_run_prune(pg_ctx* c, const float* d_base, const uint32_t* d_rows,
   if (D == 0 || D > 512) return set_err(c, PG_ERR_INVALID, "D must be in [1, 512]");
   if (mp < 2 || mp > 128) return set_err(c, PG_ERR_INVALID, "mp must be in [2, 128]");
   PG_HIP(c, hipSetDevice(c->device));
+  const pg::ScreenArgs sc = screen_args(c);
+  if (c->base16) PG_TRY(prepare_screen(c, d_base));
   PG_HIP(c, pg::launch_prune(d_base, c->p.n_base, c->p.dim, d_rows, D, d_gt,
                              d_winner, gt_k, mp, d_pruned, d_pcnt, c->stats,
-                             c->stream));
+                             c->base16 ? &sc : nullptr, c->stream));
   return PG_OK;
 }
 
@@ -520,9 +571,11 @@ int pg_run_merge(pg_ctx* c, const float* d_base, const uint32_t* d_off,
   if (cap + mp > 128) return set_err(c, PG_ERR_INVALID, "cap + mp must be <= 128");
   if (deg == 0 || deg > 128) return set_err(c, PG_ERR_INVALID, "deg must be in [1, 128]");
   PG_HIP(c, hipSetDevice(c->device));
+  const pg::ScreenArgs sc = screen_args(c);
+  if (c->base16) PG_TRY(prepare_screen(c, d_base));
   PG_HIP(c, pg::launch_merge(d_base, c->p.n_base, c->p.dim, d_off, d_srcs, cap,
                              d_pruned, d_pcnt, mp, deg, strict, d_out,
-                             c->stats, c->stream));
+                             c->stats, c->base16 ? &sc : nullptr, c->stream));
   return PG_OK;
 }
 
@@ -536,9 +589,7 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
     return set_err(c, PG_ERR_INVALID, "Kd (%u) exceeds graph degree (%u)", p.Kd,
                    graph_deg);
   PG_HIP(c, hipSetDevice(c->device));
-  if (c->base16)
-    PG_HIP(c, pg::launch_to_bf16(d_base, p.n_base, p.dim, c->dpad, c->base16,
-                                 c->stream));
+  if (c->base16) PG_TRY(prepare_screen(c, d_base));
   PG_HIP(c, pg::launch_search(d_queries, n_query, d_base, c->base16, c->dpad,
                               p.n_base, p.dim,
                               d_graph, graph_deg, p.Km, p.Kp, p.Kd, p.topk,

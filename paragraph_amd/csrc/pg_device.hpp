@@ -6,6 +6,9 @@
 
 namespace pg {
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 constexpr uint32_t kTomb = 0xFFFFFFFFu;
 constexpr uint32_t kMaskedTomb = 0x7FFFFFFFu;  // tomb after & 0x7FFFFFFF
 constexpr uint32_t kMsb = 0x80000000u;

@@ -6,6 +6,14 @@
 
 namespace pg {
 
+// Screened-path inputs of the prune / merge kernels (see pg_kernels.hip).
+struct ScreenArgs {
+  const __bf16* base16;    // bf16(x), rows padded to dpad, zero-filled
+  const float* norms;      // FP32 row norms
+  uint32_t dpad;
+  float eps_c;             // |screened - exact chain| <= eps_c * |x| * |y|
+};
+
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n,
                            unsigned long long v, hipStream_t s);
@@ -17,7 +25,8 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
                         const uint32_t* rows, uint32_t D, const uint32_t* gt,
                         const uint32_t* winner, uint32_t gt_k, uint32_t mp,
                         uint32_t* pruned, uint32_t* pcnt,
-                        unsigned long long* stats, hipStream_t s);
+                        unsigned long long* stats, const ScreenArgs* sc,
+                        hipStream_t s);
 hipError_t launch_reverse(const uint32_t* pruned, const uint32_t* pcnt,
                           uint32_t n_base, uint32_t mp, uint32_t* deg,
                           uint32_t* off, uint32_t* cursor, uint32_t* scan_tmp,
@@ -27,11 +36,13 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
                         uint32_t cap, const uint32_t* pruned,
                         const uint32_t* pcnt, uint32_t mp, uint32_t deg,
                         int strict, uint32_t* out, unsigned long long* stats,
-                        hipStream_t s);
+                        const ScreenArgs* sc, hipStream_t s);
+
 size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
                          uint32_t Kp);
-hipError_t launch_to_bf16(const float* base, uint32_t n, uint32_t dim,
-                          uint32_t dpad, void* out, hipStream_t s);
+hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
+                                 uint32_t dpad, void* out16, float* norms,
+                                 hipStream_t s);
 // base16 == nullptr: exact FP32 search; else BF16-screened search over the
 // padded BF16 copy (row stride dpad elements).
 hipError_t launch_search(const float* queries, uint32_t n_query,

@@ -142,6 +142,161 @@ __device__ uint32_t rng_select(const float* __restrict__ base, uint32_t n_base,
   return nk;
 }
 
+// Screened RNG (the BF16 path): the same decisions as rng_select, with most
+// pair distances taken from BF16 MFMA Gram tiles.  Candidates are resolved
+// in blocks of 16 (lane l%16 = block candidate): v_mfma_f32_16x16x32_bf16
+// chains give the block against each 16-wide tile of kept entries and
+// against the block itself (BF16 rows, FP32 accumulation).  A screened
+// value s decides a comparison only when the bound |s - chain| <=
+// eps_c*|c|*|k| (BF16 rounding of both rows, FP32 accumulation, the chain's
+// own error, Cauchy-Schwarz) puts both sides of d_i out of reach;
+// otherwise the exact FP32 chain is computed (counted as a fallback).  So
+// every decision equals the plain FP32 path's.
+// 1: occluded for sure, 0: not occluded for sure, -1: undecided.
+__device__ __forceinline__ int screen_cmp(float s, float d, float e) {
+  if (s + e < d) return 1;
+  if (s - e >= d) return 0;
+  return -1;
+}
+
+template <bool kStopIds, int NCH, int NKT>
+__device__ uint32_t rng_select_screened(
+    const float* __restrict__ base, const ScreenArgs& sc, uint32_t n_base,
+    uint32_t dim, const uint64_t* skey, const uint32_t* sid, uint32_t n_scan,
+    uint32_t cap, uint32_t* kept, float* kept_norm, uint32_t nk, float* gs,
+    unsigned long long* pair_total, unsigned long long* fb_total) {
+  constexpr uint32_t GW = NKT * 16 + 16;  // gs row width
+  const uint32_t lane = lane_id();
+  const uint32_t m = lane & 15u, kq = lane >> 4;
+  const uint32_t nch = sc.dpad / 32;
+  unsigned long long pairs = 0, fb = 0;
+  bool stop = false;
+  for (uint32_t blk = 1; blk < n_scan && !stop && nk < cap; blk += 16) {
+    const uint32_t pos = blk + m;
+    const bool in = pos < n_scan;
+    uint32_t my_id = kTomb;
+    float my_d = FLT_MAX;
+    if (in) {
+      const uint64_t k = skey[pos];
+      my_id = sid[uint32_t(k)];
+      my_d = from_ordered(uint32_t(k >> 32));
+    }
+    uint32_t lim = 16;
+    if (kStopIds) {
+      const uint64_t sm = __ballot(lane < 16 && in && is_stop_id(my_id, n_base));
+      if (sm) {
+        lim = __ffsll((long long)sm) - 1;
+        stop = true;
+      }
+    }
+    const bool active = in && m < lim;
+    const uint32_t nk0 = nk;
+    // ---- Gram tiles
+    bf16x8 a[NCH];
+    {
+      const bf16x8* arow = reinterpret_cast<const bf16x8*>(
+          sc.base16 + size_t(active ? my_id : 0u) * sc.dpad + 8u * kq);
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        if (active && uint32_t(c) < nch) {
+          a[c] = arow[4 * c];
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) a[c][j] = (__bf16)0.0f;
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NKT; ++t) {
+      if (16u * t < nk0) {
+        const uint32_t kidx = 16u * t + m;
+        const bool kv = kidx < nk0;
+        const bf16x8* brow = reinterpret_cast<const bf16x8*>(
+            sc.base16 + size_t(kv ? kept[kidx] : 0u) * sc.dpad + 8u * kq);
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          bf16x8 b;
+          if (kv && uint32_t(c) < nch) {
+            b = brow[4 * c];
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[j] = (__bf16)0.0f;
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c], b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gs[(4u * kq + j) * GW + 16u * t + m] = acc[j];
+      }
+    }
+    {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c], a[c], acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        gs[(4u * kq + j) * GW + NKT * 16u + m] = acc[j];
+    }
+    __syncthreads();
+    // ---- resolution: lane i < 16 owns block candidate i
+    const bool own = lane < 16 && active;
+    const float my_norm = own ? sc.norms[my_id] : 0.0f;
+    const float* own_row = base + size_t(own ? my_id : 0u) * dim;
+    bool occl = false;
+    uint32_t my_pairs = 0, my_fb = 0;
+    if (own) {
+      for (uint32_t k = 0; k < nk0; ++k) {
+        ++my_pairs;
+        int dec = screen_cmp(-gs[m * GW + k], my_d,
+                             sc.eps_c * my_norm * kept_norm[k]);
+        if (dec < 0) {
+          ++my_fb;
+          dec = chain_dist(base + size_t(kept[k]) * dim, own_row, dim) < my_d;
+        }
+        if (dec) {
+          occl = true;
+          break;
+        }
+      }
+    }
+    int last = -1, cut = 16;
+    while (true) {
+      const uint64_t msk = __ballot(own && !occl && int(lane) > last);
+      if (msk == 0) break;
+      const int j = __ffsll((long long)msk) - 1;
+      const uint32_t kid = __shfl(my_id, j, 64);
+      const float kn = __shfl(my_norm, j, 64);
+      if (lane == 0) {
+        kept[nk] = kid;
+        kept_norm[nk] = kn;
+      }
+      ++nk;
+      last = j;
+      if (nk == cap) {
+        cut = j;
+        break;
+      }
+      if (own && !occl && int(lane) > j) {
+        ++my_pairs;
+        int dec = screen_cmp(-gs[m * GW + NKT * 16u + j], my_d,
+                             sc.eps_c * my_norm * kn);
+        if (dec < 0) {
+          ++my_fb;
+          dec = chain_dist(base + size_t(kid) * dim, own_row, dim) < my_d;
+        }
+        if (dec) occl = true;
+      }
+    }
+    pairs += wave_sum<unsigned long long>((own && int(lane) <= cut) ? my_pairs : 0u);
+    fb += wave_sum<unsigned long long>(own ? my_fb : 0u);
+    __syncthreads();
+  }
+  *pair_total += pairs;
+  *fb_total += fb;
+  return nk;
+}
+
 // --------------------------------------------------------- K2: prune (pruned)
 // fusion_prune_reverse_kernel_v0 (Gbuilder.cuh:1105-1244), one wave per node.
 // Candidates: rows[b][0..D) or, in top-1 mode (rows == nullptr), the row the
@@ -151,16 +306,21 @@ __device__ uint32_t rng_select(const float* __restrict__ base, uint32_t n_base,
 // stable bubble sort; the node is skipped if the first entry is a stop id;
 // then strict RNG up to mp kept.  The reverse edges are built afterwards
 // from the pruned lists (reverse_* kernels).
-template <uint32_t DP2>
+// NCH > 0 selects the screened RNG (BF16 MFMA tiles, NCH 32-wide chunks,
+// NKT 16-wide kept tiles); NCH == 0 is the plain FP32 path.
+template <uint32_t DP2, int NCH, int NKT>
 __global__ __launch_bounds__(64) void prune_kernel(
     const float* __restrict__ base, uint32_t n_base, uint32_t dim,
     const uint32_t* __restrict__ rows, uint32_t D,
     const uint32_t* __restrict__ gt, const uint32_t* __restrict__ winner,
     uint32_t gt_k, uint32_t mp, uint32_t* __restrict__ pruned,
-    uint32_t* __restrict__ pcnt, unsigned long long* __restrict__ stats) {
+    uint32_t* __restrict__ pcnt, unsigned long long* __restrict__ stats,
+    ScreenArgs sc) {
   __shared__ uint64_t skey[DP2];
   __shared__ uint32_t sid[DP2];
   __shared__ uint32_t kept[128];
+  __shared__ float kept_norm[NCH ? 128 : 1];
+  __shared__ float gs[NCH ? 16 * (NKT * 16 + 16) : 1];
   const uint32_t b = blockIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t w = rows ? 0u : winner[b];
@@ -187,13 +347,22 @@ __global__ __launch_bounds__(64) void prune_kernel(
   }
   __syncthreads();
   bitonic_sort(skey, DP2);
-  unsigned long long pairs = 0;
+  unsigned long long pairs = 0, fbk = 0;
   uint32_t nk = 0;
   const uint32_t id0 = sid[uint32_t(skey[0])];
   if (!is_stop_id(id0, n_base)) {
-    if (lane == 0) kept[0] = id0;
+    if (lane == 0) {
+      kept[0] = id0;
+      if (NCH) kept_norm[0] = sc.norms[id0];
+    }
     __syncthreads();
-    nk = rng_select<true>(base, n_base, dim, skey, sid, D, mp, kept, 1, &pairs);
+    if constexpr (NCH == 0)
+      nk = rng_select<true>(base, n_base, dim, skey, sid, D, mp, kept, 1,
+                            &pairs);
+    else
+      nk = rng_select_screened<true, NCH, NKT>(base, sc, n_base, dim, skey,
+                                               sid, D, mp, kept, kept_norm, 1,
+                                               gs, &pairs, &fbk);
   }
   for (uint32_t j = lane; j < nk; j += kWave) pruned[size_t(b) * mp + j] = kept[j];
   if (lane == 0) pcnt[b] = nk;
@@ -201,6 +370,7 @@ __global__ __launch_bounds__(64) void prune_kernel(
   if (lane == 0) {
     stat_add(stats, ST_PRUNE_DIST, nd);
     stat_add(stats, ST_PRUNE_PAIRS, pairs);
+    stat_add(stats, ST_SCREEN_FALLBACK, fbk);
   }
 }
 
@@ -326,17 +496,20 @@ __global__ void scan_add_kernel(uint32_t* __restrict__ out, uint32_t n,
 // (Gbuilder.cuh:1559-1594), tomb fill.
 constexpr uint32_t kSegChunk = 512;
 
+template <int NCH, int NKT>
 __global__ __launch_bounds__(64) void merge_kernel(
     const float* __restrict__ base, uint32_t n_base, uint32_t dim,
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ srcs,
     uint32_t cap, const uint32_t* __restrict__ pruned,
     const uint32_t* __restrict__ pcnt, uint32_t mp, uint32_t deg,
     int strict, uint32_t* __restrict__ out,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, ScreenArgs sc) {
   __shared__ uint32_t buf[2 * kSegChunk];
   __shared__ uint64_t skey[128];
   __shared__ uint32_t sid[128];
   __shared__ uint32_t kept[128];
+  __shared__ float kept_norm[NCH ? 128 : 1];
+  __shared__ float gs[NCH ? 16 * (NKT * 16 + 16) : 1];
   __shared__ uint8_t added[128];
   const uint32_t c = blockIdx.x;
   const uint32_t lane = lane_id();
@@ -383,11 +556,20 @@ __global__ __launch_bounds__(64) void merge_kernel(
     return;
   }
   bitonic_sort(skey, 128u);
-  unsigned long long pairs = 0;
-  if (lane == 0) kept[0] = sid[uint32_t(skey[0])];
+  unsigned long long pairs = 0, fbk = 0;
+  if (lane == 0) {
+    kept[0] = sid[uint32_t(skey[0])];
+    if (NCH) kept_norm[0] = sc.norms[kept[0]];
+  }
   __syncthreads();
-  uint32_t nk =
-      rng_select<false>(base, n_base, dim, skey, sid, n, deg, kept, 1, &pairs);
+  uint32_t nk;
+  if constexpr (NCH == 0)
+    nk = rng_select<false>(base, n_base, dim, skey, sid, n, deg, kept, 1,
+                           &pairs);
+  else
+    nk = rng_select_screened<false, NCH, NKT>(base, sc, n_base, dim, skey, sid,
+                                              n, deg, kept, kept_norm, 1, gs,
+                                              &pairs, &fbk);
   if (!strict && nk < deg) {
     // mark the first list slot equal to each kept id, then append the
     // unmarked entries in sorted order (duplicates can be appended again).
@@ -418,6 +600,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
   if (lane == 0) {
     stat_add(stats, ST_MERGE_DIST, n);
     stat_add(stats, ST_MERGE_PAIRS, pairs);
+    stat_add(stats, ST_SCREEN_FALLBACK, fbk);
     if (seg > cap) stat_add(stats, ST_REV_OVERFLOW, 1);
   }
 }
@@ -460,6 +643,7 @@ __device__ __forceinline__ bool visit_insert(uint32_t* tab, uint32_t tsize,
 // pairs have their rows in flight together.
 template <int TQ, int U>
 struct EvalF32 {
+  static constexpr bool kExactVisited = true;
   float qv[TQ];
   const float* __restrict__ base;
   uint32_t dim;
@@ -520,11 +704,9 @@ struct EvalF32 {
 // 32c + 8(l/16) .. +7 of chunk c); the query, broadcast to all 16 columns,
 // is the B operand; FP32 accumulation over NCH chunks.  Lane l of the result
 // holds rows 4(l/16) .. +3.  G groups of 16 are in flight together.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
 template <int NCH, int G>
 struct EvalBF16 {
+  static constexpr bool kExactVisited = false;
   bf16x8 qf[NCH];
   const __bf16* __restrict__ base16;
   uint32_t dim, dpad, nch;
@@ -609,6 +791,7 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
     mv[r] = idx < m ? tail[todo[idx]] : kPadKey;
   }
   reg_bitonic_sort<R>(mv);
+  __syncthreads();  // ms reuses the candidate / todo slots read above
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t idx = lane * R + r;
@@ -659,16 +842,18 @@ __global__ __launch_bounds__(64) void search_kernel(
     const uint32_t* __restrict__ graph, uint32_t gdeg, uint32_t Km,
     uint32_t Kp, uint32_t Kd, uint32_t topk, uint32_t reset_iter,
     uint32_t entry_seed, uint32_t max_iters, uint32_t tsize, uint32_t lp2,
-    uint32_t* __restrict__ out, unsigned long long* __restrict__ stats) {
+    uint32_t kd_shift, uint32_t* __restrict__ out,
+    unsigned long long* __restrict__ stats) {
   extern __shared__ __align__(16) uint8_t smem[];
   const uint32_t kpkd = Kp * Kd;
   uint32_t* tab = reinterpret_cast<uint32_t*>(smem);
   uint64_t* keys =
       reinterpret_cast<uint64_t*>(smem + ((size_t(tsize) * 4 + 15) & ~size_t(15)));
-  uint64_t* ms = keys + lp2;
-  uint32_t* cand = reinterpret_cast<uint32_t*>(ms + kpkd);
+  uint32_t* cand = reinterpret_cast<uint32_t*>(keys + lp2);
   uint32_t* todo = cand + kpkd;
   uint32_t* explore = todo + kpkd;
+  // sorted misses of the merge step: reuses cand + todo (2*kpkd words)
+  uint64_t* ms = reinterpret_cast<uint64_t*>(cand);
   const uint32_t q = blockIdx.x;
   const uint32_t lane = lane_id();
   const bool fast_merge = Km <= 256 && kpkd <= 256;
@@ -694,6 +879,14 @@ __global__ __launch_bounds__(64) void search_kernel(
     __syncthreads();
   };
   reset_table();
+  // Exact evaluators keep the reference's visited set (reset every
+  // reset_iter iterations).  The screened evaluator treats the table as a
+  // cache: it is cleared and re-seeded with the top-Km ids only when the
+  // next iteration could push it past 3/4 full.  The top-Km ids are always
+  // in it, so no id enters the list twice; a dropped id can only be
+  // evaluated again.
+  const uint32_t cache_limit = tsize - tsize / 4;
+  uint32_t cache_count = Km;
   uint32_t iter = 0, total_iter = 0;
   bool capped = false;
   while (true) {
@@ -723,9 +916,14 @@ __global__ __launch_bounds__(64) void search_kernel(
     }
     for (uint32_t e = found + lane; e < Kp; e += kWave) explore[e] = kTomb;
     __syncthreads();
+    if (!Eval::kExactVisited && cache_count + kpkd > cache_limit) {
+      reset_table();
+      cache_count = Km;
+    }
     // neighbour ids of the expanded nodes (first Kd of each graph row)
     for (uint32_t t = lane; t < kpkd; t += kWave) {
-      const uint32_t e = t / Kd, j = t - e * Kd;
+      const uint32_t e = kd_shift < 32u ? (t >> kd_shift) : t / Kd;
+      const uint32_t j = t - e * Kd;
       const uint32_t eid = explore[e];
       cand[t] = eid == kTomb ? kTomb : graph[size_t(eid) * gdeg + j];
     }
@@ -747,6 +945,7 @@ __global__ __launch_bounds__(64) void search_kernel(
     __syncthreads();
     ev.eval(cand, todo, ntodo, tail);
     ndist += ntodo;
+    cache_count += ntodo;
     __syncthreads();
     const bool last_allowed = total_iter + 1 >= max_iters;
     bool final_sort = true;
@@ -760,7 +959,7 @@ __global__ __launch_bounds__(64) void search_kernel(
     }
     if (final_sort) bitonic_sort(keys, lp2);
     ++total_iter;
-    if (++iter == reset_iter) {
+    if (Eval::kExactVisited && ++iter == reset_iter) {
       reset_table();
       iter = 0;
     }
@@ -817,17 +1016,41 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
                         const uint32_t* rows, uint32_t D, const uint32_t* gt,
                         const uint32_t* winner, uint32_t gt_k, uint32_t mp,
                         uint32_t* pruned, uint32_t* pcnt,
-                        unsigned long long* stats, hipStream_t s) {
+                        unsigned long long* stats, const ScreenArgs* sc,
+                        hipStream_t s) {
   if (n_base == 0) return hipSuccess;
-#define PG_PRUNE(DP)                                                     \
-  prune_kernel<DP><<<n_base, 64, 0, s>>>(base, n_base, dim, rows, D, gt, \
-                                         winner, gt_k, mp, pruned, pcnt, \
-                                         stats)
-  if (D <= 64) PG_PRUNE(64);
-  else if (D <= 128) PG_PRUNE(128);
-  else if (D <= 256) PG_PRUNE(256);
-  else if (D <= 512) PG_PRUNE(512);
-  else return hipErrorInvalidValue;
+  const ScreenArgs scv = sc ? *sc : ScreenArgs{nullptr, nullptr, 0, 0.0f};
+#define PG_PRUNE(DP, NC, NK)                                                 \
+  prune_kernel<DP, NC, NK><<<n_base, 64, 0, s>>>(base, n_base, dim, rows, D, \
+                                                 gt, winner, gt_k, mp,       \
+                                                 pruned, pcnt, stats, scv)
+#define PG_PRUNE_D(NC, NK)                    \
+  do {                                        \
+    if (D <= 128) PG_PRUNE(128, NC, NK);      \
+    else if (D <= 256) PG_PRUNE(256, NC, NK); \
+    else PG_PRUNE(512, NC, NK);               \
+  } while (0)
+#define PG_PRUNE_K(NC)                        \
+  do {                                        \
+    if (mp <= 32) PG_PRUNE_D(NC, 2);          \
+    else if (mp <= 64) PG_PRUNE_D(NC, 4);     \
+    else PG_PRUNE_D(NC, 8);                   \
+  } while (0)
+  if (D > 512) return hipErrorInvalidValue;
+  if (sc == nullptr) {
+    if (D <= 64) PG_PRUNE(64, 0, 1);
+    else if (D <= 128) PG_PRUNE(128, 0, 1);
+    else if (D <= 256) PG_PRUNE(256, 0, 1);
+    else PG_PRUNE(512, 0, 1);
+  } else if (sc->dpad <= 256) {
+    PG_PRUNE_K(8);
+  } else if (sc->dpad <= 512) {
+    PG_PRUNE_K(16);
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef PG_PRUNE_K
+#undef PG_PRUNE_D
 #undef PG_PRUNE
   return hipGetLastError();
 }
@@ -858,17 +1081,38 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
                         uint32_t cap, const uint32_t* pruned,
                         const uint32_t* pcnt, uint32_t mp, uint32_t deg,
                         int strict, uint32_t* out, unsigned long long* stats,
-                        hipStream_t s) {
+                        const ScreenArgs* sc, hipStream_t s) {
   if (n_base == 0) return hipSuccess;
-  merge_kernel<<<n_base, 64, 0, s>>>(base, n_base, dim, off, srcs, cap,
-                                     pruned, pcnt, mp, deg, strict, out, stats);
+  const ScreenArgs scv = sc ? *sc : ScreenArgs{nullptr, nullptr, 0, 0.0f};
+#define PG_MERGE(NC, NK)                                                   \
+  merge_kernel<NC, NK><<<n_base, 64, 0, s>>>(base, n_base, dim, off, srcs, \
+                                             cap, pruned, pcnt, mp, deg,   \
+                                             strict, out, stats, scv)
+#define PG_MERGE_K(NC)                    \
+  do {                                    \
+    if (deg <= 32) PG_MERGE(NC, 2);       \
+    else if (deg <= 64) PG_MERGE(NC, 4);  \
+    else PG_MERGE(NC, 8);                 \
+  } while (0)
+  if (sc == nullptr || !strict) {
+    PG_MERGE(0, 1);  // the non-strict fill runs on the plain kernel only
+  } else if (sc->dpad <= 256) {
+    PG_MERGE_K(8);
+  } else if (sc->dpad <= 512) {
+    PG_MERGE_K(16);
+  } else {
+    return hipErrorInvalidValue;
+  }
+#undef PG_MERGE_K
+#undef PG_MERGE
   return hipGetLastError();
 }
+
 
 size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
                          uint32_t Kp) {
   return ((size_t(table_size) * 4 + 15) & ~size_t(15)) + size_t(lp2) * 8 +
-         size_t(kpkd) * 16 + size_t(Kp) * 4 + 16;
+         size_t(kpkd) * 8 + size_t(Kp) * 4 + 16;
 }
 
 template <class Eval>
@@ -884,9 +1128,12 @@ static hipError_t launch_search_eval(
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&search_kernel<Eval>),
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             int(smem));
+  uint32_t kd_shift = 0xFFu;
+  for (uint32_t sh = 0; sh < 31; ++sh)
+    if ((1u << sh) == Kd) kd_shift = sh;
   search_kernel<Eval><<<n_query, 64, smem, s>>>(
       ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,
-      entry_seed, max_iters, table_size, lp2, out, stats);
+      entry_seed, max_iters, table_size, lp2, kd_shift, out, stats);
   return hipGetLastError();
 }
 
@@ -973,26 +1220,34 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
   return hipErrorInvalidValue;
 }
 
-// Row-major FP32 base -> BF16 copy with rows zero-padded to dpad (round to
-// nearest even), one thread per output element pair.
-__global__ void to_bf16_kernel(const float* __restrict__ base, uint64_t n,
-                               uint32_t dim, uint32_t dpad,
-                               __bf16* __restrict__ out) {
-  const uint64_t total = n * dpad;
-  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < total;
-       i += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t r = i / dpad;
-    const uint32_t c = uint32_t(i - r * dpad);
-    out[i] = (__bf16)(c < dim ? base[r * dim + c] : 0.0f);
+// Screened-path preparation, one wave per row: the BF16 copy of the row
+// (round to nearest even, zero-padded to dpad) and its FP32 norm.
+__global__ __launch_bounds__(256) void prepare_screen_kernel(
+    const float* __restrict__ base, uint32_t n, uint32_t dim, uint32_t dpad,
+    __bf16* __restrict__ out16, float* __restrict__ norms) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wpb = blockDim.x / 64u;
+  for (uint32_t r = blockIdx.x * wpb + threadIdx.x / 64u; r < n;
+       r += gridDim.x * wpb) {
+    const float* x = base + size_t(r) * dim;
+    float ss = 0.0f;
+    for (uint32_t c = lane; c < dpad; c += 64u) {
+      const float v = c < dim ? x[c] : 0.0f;
+      out16[size_t(r) * dpad + c] = (__bf16)v;
+      ss = __builtin_fmaf(v, v, ss);
+    }
+    ss = wave_sum(ss);
+    if (lane == 0) norms[r] = __builtin_sqrtf(ss);
   }
 }
 
-hipError_t launch_to_bf16(const float* base, uint32_t n, uint32_t dim,
-                          uint32_t dpad, void* out, hipStream_t s) {
-  const uint64_t total = uint64_t(n) * dpad;
-  if (total == 0) return hipSuccess;
-  to_bf16_kernel<<<grid_for(total, 256), 256, 0, s>>>(
-      base, n, dim, dpad, reinterpret_cast<__bf16*>(out));
+hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
+                                 uint32_t dpad, void* out16, float* norms,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = (n + 3) / 4 < 65536u ? (n + 3) / 4 : 65536u;
+  prepare_screen_kernel<<<blocks, 256, 0, s>>>(
+      base, n, dim, dpad, reinterpret_cast<__bf16*>(out16), norms);
   return hipGetLastError();
 }
 

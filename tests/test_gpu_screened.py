@@ -128,3 +128,78 @@ def test_screened_prune_equals_fp32_prune(cuda):
     np.testing.assert_array_equal(_host(pc_d), pc)
     mask = np.arange(32)[None, :] < pc[:, None]
     np.testing.assert_array_equal(_host(pr_d)[mask], pr[mask])
+
+
+def _stage_inputs(n, dim, seed=0, integer=False):
+    from paragraph_amd import datagen
+    if integer:
+        # small-integer rows with many duplicates: exact distance ties, so
+        # the screen cannot decide and the exact fallback must run
+        rng = np.random.default_rng(seed)
+        protos = rng.integers(-2, 3, (n // 8, dim)).astype(np.float32)
+        base = protos[rng.integers(0, protos.shape[0], n)]
+        return np.ascontiguousarray(base)
+    return datagen.base_vectors(n, dim, 64)
+
+
+@pytest.mark.parametrize("integer", [False, True])
+@pytest.mark.parametrize("dim,mp", [(200, 32), (512, 64)])
+def test_screened_prune_and_merge_equal_oracle(integer, dim, mp, cuda):
+    """BF16 screen + exact fallback must reproduce the FP32 decisions exactly,
+    including on data full of exact ties (fallbacks exercised)."""
+    import torch
+    n = 1200
+    base = _stage_inputs(n, dim, integer=integer)
+    rng = np.random.default_rng(dim + int(integer))
+    rows = rng.integers(0, n, (n, 256)).astype(np.uint32)
+    rows[rng.random(rows.shape) < 0.05] = 0x7FFFFFFF
+    p = _native.make_params(n, dim, n, mp, 128, precision=_native.PG_BF16)
+    base_d = _dev(base, torch)
+    pr_d = torch.full((n, mp), -1, dtype=torch.int32, device=cuda)
+    pc_d = torch.zeros(n, dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.run_prune_rows(base_d, _dev(rows, torch), mp, pr_d, pc_d)
+        ctx.sync()
+        pr, pc = cpu_ref.prune(base, rows, mp)
+        np.testing.assert_array_equal(_host(pc_d), pc)
+        mask = np.arange(mp)[None, :] < pc[:, None]
+        np.testing.assert_array_equal(_host(pr_d)[mask], pr[mask])
+        off, srcs = cpu_ref.reverse_csr(pr, pc)
+        cap = 126 - mp
+        out_d = torch.empty((n, mp), dtype=torch.int32, device=cuda)
+        ctx.run_merge(base_d, _dev(off, torch),
+                      _dev(srcs if len(srcs) else np.zeros(1, np.uint32),
+                           torch), cap, pr_d, pc_d, mp, True, out_d)
+        ctx.sync()
+        st = ctx.stats()
+    out = cpu_ref.merge(base, off, srcs, cap, pr, pc, mp, True)
+    np.testing.assert_array_equal(_host(out_d), out)
+    if integer:
+        assert st.screen_fallback > 0
+
+
+def test_screened_whole_stage_matches_oracle_given_its_search(cuda):
+    """In the screened build only the search differs from the FP32 path:
+    running the oracle's prune / merge on the screened search rows gives the
+    screened link graph exactly (round 1)."""
+    import torch
+    from paragraph_amd import datagen
+    n, dim, M, L = 1500, 200, 32, 64
+    w = datagen.make_workload(n, dim, 64, device="cpu")
+    gt = np.ascontiguousarray(w["gt"].view(np.uint32))
+    p = _native.make_params(n, dim, n, M, L, precision=_native.PG_BF16,
+                            rounds=1)
+    base_d, gt_d = _dev(w["base"], torch), _dev(gt, torch)
+    proj_d = torch.empty((n, 16), dtype=torch.int32, device=cuda)
+    link_d = torch.empty((n, M), dtype=torch.int32, device=cuda)
+    res_d = torch.empty((n, p.topk), dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.stage_device(base_d, gt_d, proj_d, link_d)
+        ctx.wait()
+        ctx.run_search(base_d, base_d, proj_d, res_d)
+        ctx.sync()
+    res = _host(res_d)
+    pr, pc = cpu_ref.prune(w["base"], res, M)
+    off, srcs = cpu_ref.reverse_csr(pr, pc)
+    link = cpu_ref.merge(w["base"], off, srcs, 126 - M, pr, pc, M, True)
+    np.testing.assert_array_equal(_host(link_d), link)
