@@ -150,6 +150,10 @@ def main():
                     help="perf-only knobs for the FP32 build")
     ap.add_argument("--tune-flags-bf16", type=int, default=0,
                     help="perf-only knobs for the BF16 build")
+    ap.add_argument("--bf16-params", default="",
+                    help="overrides of the BF16 build's knobs, e.g. "
+                         "'Kp=16,table_size=2048' (beam knobs change recall; "
+                         "the FP32 build always uses the reference mapping)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
@@ -202,9 +206,12 @@ def main():
         if prec == "fp32":
             p = p32
         else:
+            ov = dict(kv.split("=") for kv in args.bf16_params.split(",")
+                      if kv)
             p = _native.make_params(n, dim, n, M, L,
                                     precision=_native.PG_BF16,
-                                    tune_flags=args.tune_flags_bf16)
+                                    tune_flags=args.tune_flags_bf16,
+                                    **{k: int(v) for k, v in ov.items()})
         proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
         link_d = torch.empty((n, M), dtype=torch.int32, device=dev)
         ctx = _native.Context(p)

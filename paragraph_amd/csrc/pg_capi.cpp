@@ -307,6 +307,7 @@ int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len) {
     return fail("precision must be PG_FP32 or PG_BF16 in this build");
   if (p->precision == PG_BF16 && p->dim > 512)
     return fail("the BF16-screened path supports dim <= 512");
+
   if (msg && msg_len) msg[0] = 0;
   return PG_OK;
 }

@@ -58,7 +58,8 @@ __device__ __forceinline__ bool is_stop_id(uint32_t id, uint32_t n) {
 }
 
 // Sequential FMA chain over the whole row (Gbuilder.cuh:785-797), negated.
-// `x` is the wave-uniform row, `y` the per-lane row.
+// Loads are issued 8 float4 at a time per operand so that one lane keeps
+// 16 row segments in flight; the FMAs stay in index order.
 __device__ __forceinline__ float chain_dist(const float* __restrict__ x,
                                             const float* __restrict__ y,
                                             uint32_t dim) {
@@ -67,9 +68,24 @@ __device__ __forceinline__ float chain_dist(const float* __restrict__ x,
     const float4* x4 = reinterpret_cast<const float4*>(x);
     const float4* y4 = reinterpret_cast<const float4*>(y);
     const uint32_t n4 = dim >> 2;
-#pragma unroll 4
-    for (uint32_t i = 0; i < n4; ++i) {
-      float4 a = x4[i], b = y4[i];
+    uint32_t i = 0;
+    for (; i + 8 <= n4; i += 8) {
+      float4 a[8], b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        a[u] = x4[i + u];
+        b[u] = y4[i + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        s = __builtin_fmaf(a[u].x, b[u].x, s);
+        s = __builtin_fmaf(a[u].y, b[u].y, s);
+        s = __builtin_fmaf(a[u].z, b[u].z, s);
+        s = __builtin_fmaf(a[u].w, b[u].w, s);
+      }
+    }
+    for (; i < n4; ++i) {
+      const float4 a = x4[i], b = y4[i];
       s = __builtin_fmaf(a.x, b.x, s);
       s = __builtin_fmaf(a.y, b.y, s);
       s = __builtin_fmaf(a.z, b.z, s);

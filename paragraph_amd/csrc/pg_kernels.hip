@@ -323,10 +323,14 @@ __global__ __launch_bounds__(64) void prune_kernel(
   __shared__ float gs[NCH ? 16 * (NKT * 16 + 16) : 1];
   const uint32_t b = blockIdx.x;
   const uint32_t lane = lane_id();
+  __shared__ uint16_t vpos[DP2];
   const uint32_t w = rows ? 0u : winner[b];
   const float* xb = base + size_t(b) * dim;
-  uint32_t nvalid = 0;
-  for (uint32_t j = lane; j < DP2; j += kWave) {
+  // ids; invalid slots get FLT_MAX; valid positions are compacted so that
+  // every lane of the distance pass has a real candidate
+  uint32_t nv = 0;
+  for (uint32_t j0 = 0; j0 < DP2; j0 += kWave) {
+    const uint32_t j = j0 + lane;
     uint32_t id = kTomb;
     if (j < D) {
       if (rows) {
@@ -338,12 +342,16 @@ __global__ __launch_bounds__(64) void prune_kernel(
     sid[j] = id;
     const bool valid =
         j < D && !(id == kTomb || id == b || id == kMaskedTomb || id >= n_base);
-    float d = FLT_MAX;
-    if (valid) {
-      d = chain_dist(xb, base + size_t(id) * dim, dim);
-      ++nvalid;
-    }
-    skey[j] = j < D ? ((uint64_t(ordered_bits(d)) << 32) | j) : kPadKey;
+    skey[j] = j < D ? ((uint64_t(ordered_bits(FLT_MAX)) << 32) | j) : kPadKey;
+    const uint64_t vm = __ballot(valid);
+    if (valid) vpos[nv + __popcll(vm & lanemask_lt())] = uint16_t(j);
+    nv += __popcll(vm);
+  }
+  __syncthreads();
+  for (uint32_t v = lane; v < nv; v += kWave) {
+    const uint32_t j = vpos[v];
+    const float d = chain_dist(xb, base + size_t(sid[j]) * dim, dim);
+    skey[j] = (uint64_t(ordered_bits(d)) << 32) | j;
   }
   __syncthreads();
   bitonic_sort(skey, DP2);
@@ -366,9 +374,8 @@ __global__ __launch_bounds__(64) void prune_kernel(
   }
   for (uint32_t j = lane; j < nk; j += kWave) pruned[size_t(b) * mp + j] = kept[j];
   if (lane == 0) pcnt[b] = nk;
-  const unsigned long long nd = wave_sum<unsigned long long>(nvalid);
   if (lane == 0) {
-    stat_add(stats, ST_PRUNE_DIST, nd);
+    stat_add(stats, ST_PRUNE_DIST, nv);
     stat_add(stats, ST_PRUNE_PAIRS, pairs);
     stat_add(stats, ST_SCREEN_FALLBACK, fbk);
   }
@@ -382,12 +389,12 @@ __global__ void reverse_count_kernel(const uint32_t* __restrict__ pruned,
                                      const uint32_t* __restrict__ pcnt,
                                      uint32_t n_base, uint32_t mp,
                                      uint32_t* __restrict__ deg) {
-  const uint64_t total = uint64_t(n_base) * mp;
-  for (uint64_t e = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; e < total;
-       e += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t b = uint32_t(e / mp), j = uint32_t(e % mp);
-    if (j < pcnt[b]) {
-      const uint32_t c = pruned[e];
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n_base;
+       b += gridDim.x * blockDim.x) {
+    const uint32_t cnt = pcnt[b];
+    const uint32_t* row = pruned + size_t(b) * mp;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t c = row[j];
       if (c < n_base) atomicAdd(&deg[c], 1u);
     }
   }
@@ -399,12 +406,12 @@ __global__ void reverse_fill_kernel(const uint32_t* __restrict__ pruned,
                                     const uint32_t* __restrict__ off,
                                     uint32_t* __restrict__ cursor,
                                     uint32_t* __restrict__ srcs) {
-  const uint64_t total = uint64_t(n_base) * mp;
-  for (uint64_t e = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; e < total;
-       e += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t b = uint32_t(e / mp), j = uint32_t(e % mp);
-    if (j < pcnt[b]) {
-      const uint32_t c = pruned[e];
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n_base;
+       b += gridDim.x * blockDim.x) {
+    const uint32_t cnt = pcnt[b];
+    const uint32_t* row = pruned + size_t(b) * mp;
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const uint32_t c = row[j];
       if (c < n_base) {
         const uint32_t slot = atomicAdd(&cursor[c], 1u);
         srcs[off[c] + slot] = b;
@@ -644,6 +651,8 @@ __device__ __forceinline__ bool visit_insert(uint32_t* tab, uint32_t tsize,
 template <int TQ, int U>
 struct EvalF32 {
   static constexpr bool kExactVisited = true;
+  static constexpr uint32_t kQueryLdsBytes = 0;
+  bf16x8* qs;  // unused
   float qv[TQ];
   const float* __restrict__ base;
   uint32_t dim;
@@ -707,7 +716,9 @@ struct EvalF32 {
 template <int NCH, int G>
 struct EvalBF16 {
   static constexpr bool kExactVisited = false;
-  bf16x8 qf[NCH];
+  static constexpr uint32_t kQueryLdsBytes = 0;
+  bf16x8* qs;  // unused
+  bf16x8 qf[NCH];  // lane l: query elements 32c + 8(l/16) .. +7 of chunk c
   const __bf16* __restrict__ base16;
   uint32_t dim, dpad, nch;
 
@@ -779,7 +790,7 @@ struct EvalBF16 {
 // Returns true when this iteration is the last one (no unexpanded entry
 // would stay in the top Km, or the watchdog stops after it); the caller then
 // sorts the full list once, which also orders the tail the output needs.
-template <int R>
+template <int R, int HPL>
 __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
                              const uint32_t* todo, uint32_t m, uint64_t* ms,
                              bool force_final) {
@@ -798,7 +809,7 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
     if (idx < m) ms[idx] = mv[r];
   }
   __syncthreads();
-  constexpr int kHpl = 4;  // Km <= 256
+  constexpr int kHpl = HPL;  // Km <= 64 * HPL
   uint64_t hx[kHpl];
   uint32_t hp[kHpl];
   bool unexp = false;
@@ -835,7 +846,9 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
   return false;
 }
 
-template <class Eval>
+// W: list width in 64-entry slices per lane for the merge (Km and Kp*Kd
+// both <= 64*W); smaller W keeps fewer registers live.
+template <class Eval, int W>
 __global__ __launch_bounds__(64) void search_kernel(
     Eval ev, const float* __restrict__ queries, uint32_t n_query,
     uint32_t n_base,
@@ -846,9 +859,10 @@ __global__ __launch_bounds__(64) void search_kernel(
     unsigned long long* __restrict__ stats) {
   extern __shared__ __align__(16) uint8_t smem[];
   const uint32_t kpkd = Kp * Kd;
-  uint32_t* tab = reinterpret_cast<uint32_t*>(smem);
-  uint64_t* keys =
-      reinterpret_cast<uint64_t*>(smem + ((size_t(tsize) * 4 + 15) & ~size_t(15)));
+  ev.qs = reinterpret_cast<bf16x8*>(smem);
+  uint32_t* tab = reinterpret_cast<uint32_t*>(smem + Eval::kQueryLdsBytes);
+  uint64_t* keys = reinterpret_cast<uint64_t*>(
+      smem + Eval::kQueryLdsBytes + ((size_t(tsize) * 4 + 15) & ~size_t(15)));
   uint32_t* cand = reinterpret_cast<uint32_t*>(keys + lp2);
   uint32_t* todo = cand + kpkd;
   uint32_t* explore = todo + kpkd;
@@ -856,7 +870,7 @@ __global__ __launch_bounds__(64) void search_kernel(
   uint64_t* ms = reinterpret_cast<uint64_t*>(cand);
   const uint32_t q = blockIdx.x;
   const uint32_t lane = lane_id();
-  const bool fast_merge = Km <= 256 && kpkd <= 256;
+  const bool fast_merge = Km <= 64u * W && kpkd <= 64u * W;
   ev.load_query(queries, q);
   // initial list: [0, Km) = (FLT_MAX, tomb); [Km, Km+kpkd) = entry points
   for (uint32_t i = lane; i < lp2; i += kWave)
@@ -951,11 +965,14 @@ __global__ __launch_bounds__(64) void search_kernel(
     bool final_sort = true;
     if (fast_merge) {
       if (ntodo <= 64)
-        final_sort = merge_misses<1>(keys, Km, tail, todo, ntodo, ms, last_allowed);
-      else if (ntodo <= 128)
-        final_sort = merge_misses<2>(keys, Km, tail, todo, ntodo, ms, last_allowed);
+        final_sort = merge_misses<1, W>(keys, Km, tail, todo, ntodo, ms,
+                                        last_allowed);
+      else if (W <= 2 || ntodo <= 128)
+        final_sort = merge_misses<(W < 2 ? W : 2), W>(keys, Km, tail, todo,
+                                                      ntodo, ms, last_allowed);
       else
-        final_sort = merge_misses<4>(keys, Km, tail, todo, ntodo, ms, last_allowed);
+        final_sort = merge_misses<W, W>(keys, Km, tail, todo, ntodo, ms,
+                                        last_allowed);
     }
     if (final_sort) bitonic_sort(keys, lp2);
     ++total_iter;
@@ -1065,13 +1082,13 @@ hipError_t launch_reverse(const uint32_t* pruned, const uint32_t* pcnt,
   if (e != hipSuccess) return e;
   const uint64_t total = uint64_t(n_base) * mp;
   if (total == 0) return launch_fill_u32(off, uint64_t(n_base) + 1, 0u, s);
-  reverse_count_kernel<<<grid_for(total, 256), 256, 0, s>>>(pruned, pcnt,
-                                                            n_base, mp, deg);
+  reverse_count_kernel<<<grid_for(n_base, 256), 256, 0, s>>>(pruned, pcnt,
+                                                             n_base, mp, deg);
   const uint32_t nb = (n_base + kScanBlock - 1) / kScanBlock;
   scan_blocks_kernel<<<nb, 256, 0, s>>>(deg, n_base, off, scan_tmp);
   scan_sums_kernel<<<1, 1024, 0, s>>>(scan_tmp, nb, off + n_base);
   scan_add_kernel<<<(n_base + 255) / 256, 256, 0, s>>>(off, n_base, scan_tmp);
-  reverse_fill_kernel<<<grid_for(total, 256), 256, 0, s>>>(
+  reverse_fill_kernel<<<grid_for(n_base, 256), 256, 0, s>>>(
       pruned, pcnt, n_base, mp, off, cursor, srcs);
   return hipGetLastError();
 }
@@ -1124,16 +1141,23 @@ static hipError_t launch_search_eval(
     unsigned long long* stats, hipStream_t s) {
   uint32_t lp2 = 64;
   while (lp2 < Km + Kp * Kd) lp2 <<= 1;
-  const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&search_kernel<Eval>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            int(smem));
+  const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp) +
+                      Eval::kQueryLdsBytes;
   uint32_t kd_shift = 0xFFu;
   for (uint32_t sh = 0; sh < 31; ++sh)
     if ((1u << sh) == Kd) kd_shift = sh;
-  search_kernel<Eval><<<n_query, 64, smem, s>>>(
-      ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,
-      entry_seed, max_iters, table_size, lp2, kd_shift, out, stats);
+#define PG_SEARCH_W(WW)                                                     \
+  do {                                                                      \
+    (void)hipFuncSetAttribute(                                              \
+        reinterpret_cast<const void*>(&search_kernel<Eval, WW>),            \
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));             \
+    search_kernel<Eval, WW><<<n_query, 64, smem, s>>>(                      \
+        ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk,        \
+        reset_iter, entry_seed, max_iters, table_size, lp2, kd_shift, out,  \
+        stats);                                                             \
+  } while (0)
+  PG_SEARCH_W(4);
+#undef PG_SEARCH_W
   return hipGetLastError();
 }
 
@@ -1188,6 +1212,8 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
       entry_seed, max_iters, table_size, out, stats, s
   if (base16 != nullptr) {
     // screened path: gather_batch = groups of 16 candidates in flight
+    // NCH chunks of 32 (8 or 16; chunks past dpad/32 are zero and skip
+    // their loads); gather_batch >= 2 puts two 16-candidate groups in flight
     const uint32_t nch = dpad / 32;
     const bool two = gather_batch >= 2;
     if (nch <= 8) {
@@ -1198,7 +1224,6 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
       return two ? launch_bf16<16, 2>(base16, dim, dpad, PG_ARGS)
                  : launch_bf16<16, 1>(base16, dim, dpad, PG_ARGS);
     }
-    if (nch <= 32) return launch_bf16<32, 1>(base16, dim, dpad, PG_ARGS);
     return hipErrorInvalidValue;
   }
   const uint32_t tq = (dim + 31) / 32;
