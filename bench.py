@@ -146,9 +146,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--n-eval", type=int, default=10000)
-    ap.add_argument("--tune-flags", type=int, default=0,
-                    help="perf-only knobs for the FP32 build")
-    ap.add_argument("--tune-flags-bf16", type=int, default=0,
+    ap.add_argument("--tune-flags", type=int, default=None,
+                    help="perf-only knobs for the FP32 build (default: the "
+                         "library's)")
+    ap.add_argument("--tune-flags-bf16", type=int, default=None,
                     help="perf-only knobs for the BF16 build")
     ap.add_argument("--bf16-params", default="",
                     help="overrides of the BF16 build's knobs, e.g. "
@@ -188,7 +189,10 @@ def main():
 
     precs = ["fp32", "bf16"] if args.precision == "both" else [args.precision]
     results = {}
-    p32 = _native.make_params(n, dim, n, M, L, tune_flags=args.tune_flags)
+    tf32 = {} if args.tune_flags is None else {"tune_flags": args.tune_flags}
+    tf16 = {} if args.tune_flags_bf16 is None else {
+        "tune_flags": args.tune_flags_bf16}
+    p32 = _native.make_params(n, dim, n, M, L, **tf32)
     eval_ctx = None
     truth = None
     if not args.no_recall:
@@ -208,16 +212,26 @@ def main():
         else:
             ov = dict(kv.split("=") for kv in args.bf16_params.split(",")
                       if kv)
+            kw = dict(tf16)
+            kw.update({k: int(v) for k, v in ov.items()})
             p = _native.make_params(n, dim, n, M, L,
-                                    precision=_native.PG_BF16,
-                                    tune_flags=args.tune_flags_bf16,
-                                    **{k: int(v) for k, v in ov.items()})
+                                    precision=_native.PG_BF16, **kw)
         proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
         link_d = torch.empty((n, M), dtype=torch.int32, device=dev)
         ctx = _native.Context(p)
         ms = time_build(ctx, base_d, gt_d, proj_d, link_d, args.steps,
                         args.warmup, prec)
-        roof, detail = roofline_of(ctx, p, n, args.config, prec)
+        if p.tune_flags & 0x40:
+            # graph replays carry no stage events: one eager build of the
+            # same knobs gives the per-stage breakdown
+            pe = _native.PgParams.from_buffer_copy(p)
+            pe.tune_flags = p.tune_flags & ~0x40
+            with _native.Context(pe) as ectx:
+                time_build(ectx, base_d, gt_d, proj_d, link_d, 1, 0,
+                           prec + " (eager breakdown)")
+                roof, detail = roofline_of(ectx, pe, n, args.config, prec)
+        else:
+            roof, detail = roofline_of(ctx, p, n, args.config, prec)
         r = {"ms_per_step": ms, "value": n / (ms / 1000.0),
              "roofline": roof, **detail,
              "counters": {k: v for k, v in ctx.stats().as_dict().items()

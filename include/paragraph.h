@@ -73,8 +73,13 @@ typedef struct pg_params {
   uint32_t max_iters;   /* search iteration watchdog (not in the reference) */
   uint32_t precision;   /* pg_precision */
   uint32_t rerank;      /* screened paths: FP32 re-rank width (0 = topk) */
-  uint32_t tune_flags;  /* perf-only knobs, 0 = defaults; bits 0-4: search
-                           gather batch (candidate pairs in flight) */
+  uint32_t tune_flags;  /* perf-only knobs, 0 = defaults.  bits 0-4: search
+                           gather batch (FP32: candidate pairs in flight;
+                           BF16: 16-candidate groups); bit 5 (0x20):
+                           persistent search grid fed by a device work
+                           counter; bit 6 (0x40): capture the pipeline in a
+                           hipGraph once per buffer set and replay it (no
+                           per-stage times in that mode) */
   uint32_t reserved[7];
 } pg_params;
 
@@ -100,7 +105,8 @@ typedef struct pg_ctx pg_ctx;
 /* Fills *p with the project's mapping of (N, dim, M, L) onto the reference
  * knobs: link_rev = 126-M, Km = L, Kd = 16, Kp = ceil(L/16), topk = 2L,
  * reset_iter 15, table auto, 2 rounds, seed 3030517, projection 16/15/111
- * (reference values: tests/test_t2iend.cu:676-707). */
+ * (reference values: tests/test_t2iend.cu:676-707); tune_flags = persistent
+ * search grid. */
 int pg_params_init(pg_params* p, uint32_t n_base, uint32_t dim,
                    uint32_t n_query, uint32_t M, uint32_t L);
 

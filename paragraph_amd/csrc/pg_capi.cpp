@@ -56,6 +56,11 @@ struct pg_ctx {
   float eps_c = 0.0f;       // screening bound factor
   unsigned long long* stats = nullptr;
   unsigned long long* stage_cnt = nullptr;  // (kMaxStages+1) x 16 snapshots
+  uint32_t* queue = nullptr;                // persistent-search work counter
+  bool capturing = false;                   // recording a hipGraph
+  hipGraph_t graph = nullptr;               // captured pipeline
+  hipGraphExec_t graph_exec = nullptr;
+  const void* graph_key[4] = {};            // buffers the graph was built for
   // staging for the host-pointer entry point (allocated on first use)
   float* h_base_dev = nullptr;
   uint32_t* h_gt_dev = nullptr;
@@ -120,7 +125,12 @@ uint32_t effective_table_size(const pg_params& p) {
 }
 
 // Perf-only knobs packed in tune_flags (0 = library default):
-//   bits 0-4  search gather batch (candidate pairs in flight; 4, 8 or 16)
+//   bits 0-4  search gather batch (FP32: candidate pairs in flight, 4/8/16;
+//             BF16: 16-candidate groups in flight, 1/2)
+//   bit 5     persistent search grid fed by a device work counter
+//   bit 6     capture the pipeline in a hipGraph once, replay it per build
+constexpr uint32_t kTunePersistent = 0x20u;
+constexpr uint32_t kTuneGraph = 0x40u;
 uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
 
 // Bound factor of the BF16 screen against the exact FP32 chain:
@@ -149,7 +159,8 @@ uint32_t next_pow2(uint32_t x) {
 
 int record(pg_ctx* c, int idx) {
   if (idx <= kMaxStages) {
-    PG_HIP(c, hipEventRecord(c->ev[idx], c->stream));
+    // stage events are not recorded into a graph; counters still are
+    if (!c->capturing) PG_HIP(c, hipEventRecord(c->ev[idx], c->stream));
     PG_HIP(c, pg::launch_snapshot_stats(c->stats, c->stage_cnt + idx * 16,
                                         c->stream));
   }
@@ -205,7 +216,10 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
                                 p.Kp, p.Kd, p.topk, p.reset_iter,
                                 p.entry_seed, p.max_iters,
                                 effective_table_size(p), tune_gather(p),
-                                c->res, c->stats, s));
+                                c->res, c->stats,
+                                (p.tune_flags & kTunePersistent) ? c->queue
+                                                                 : nullptr,
+                                s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, p.topk, nullptr,
                                nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
@@ -221,8 +235,48 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
     graph = d_link;
     gdeg = p.M;
   }
-  c->n_stage_events = ev;
-  PG_HIP(c, hipEventRecord(c->done, s));
+  c->n_stage_events = c->capturing ? 0 : ev;
+  if (!c->capturing) PG_HIP(c, hipEventRecord(c->done, s));
+  return PG_OK;
+}
+
+void drop_graph(pg_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+}
+
+// Graph mode: the pipeline (kernels only: every clear is a fill kernel, no
+// memset node) is captured once per set of buffers and replayed.  Stage
+// events are not recorded in graph mode, so per-stage times are reported by
+// eager builds only.
+int run_pipeline_graph(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
+                       uint32_t* d_proj, uint32_t* d_link) {
+  const void* key[4] = {d_base, d_gt, d_proj, d_link};
+  if (!c->graph_exec || std::memcmp(key, c->graph_key, sizeof(key)) != 0) {
+    drop_graph(c);
+    PG_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    c->capturing = true;
+    const int rc = run_pipeline(c, d_base, d_gt, d_proj, d_link);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != PG_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess)
+      return set_err(c, PG_ERR_HIP, "hipStreamEndCapture: %s",
+                     hipGetErrorString(e));
+    c->graph = g;
+    PG_HIP(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr,
+                                  0));
+    std::memcpy(c->graph_key, key, sizeof(key));
+  }
+  PG_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+  PG_HIP(c, hipEventRecord(c->done, c->stream));
+  c->n_stage_events = 0;
   return PG_OK;
 }
 
@@ -254,6 +308,7 @@ int pg_params_init(pg_params* p, uint32_t n_base, uint32_t dim,
   p->max_iters = 100000;
   p->precision = PG_FP32;
   p->rerank = 0;
+  p->tune_flags = kTunePersistent;  // measured faster on MI355X
   return PG_OK;
 }
 
@@ -372,7 +427,8 @@ pg_ctx* pg_create(int device, const pg_params* p) {
                   N * ((p->dim + 31) / 32 * 32), "base_bf16") ||
         dev_alloc(c, &c->norms, N, "norms"))) ||
       dev_alloc(c, &c->stats, kStatCount, "stats") ||
-      dev_alloc(c, &c->stage_cnt, (kMaxStages + 1) * 16ull, "stage_cnt"))
+      dev_alloc(c, &c->stage_cnt, (kMaxStages + 1) * 16ull, "stage_cnt") ||
+      dev_alloc(c, &c->queue, 1, "queue"))
     return fail(0);
   if (pg::launch_fill_u64(c->stats, kStatCount, 0ull, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -398,8 +454,10 @@ void pg_destroy(pg_ctx* c) {
   if (c->base16) (void)hipFree(c->base16);
   c->base16 = nullptr;
   dev_free(c->norms);
+  drop_graph(c);
   dev_free(c->stats);
   dev_free(c->stage_cnt);
+  dev_free(c->queue);
   dev_free(c->h_base_dev);
   dev_free(c->h_gt_dev);
   dev_free(c->h_proj_dev);
@@ -418,6 +476,8 @@ int pg_gpu_stage_device_async(pg_ctx* c, const float* d_base,
   if (!d_base || !d_gt || !d_proj || !d_link)
     return set_err(c, PG_ERR_INVALID, "NULL buffer");
   PG_HIP(c, hipSetDevice(c->device));
+  if (c->p.tune_flags & kTuneGraph)
+    return run_pipeline_graph(c, d_base, d_gt, d_proj, d_link);
   return run_pipeline(c, d_base, d_gt, d_proj, d_link);
 }
 
@@ -440,8 +500,12 @@ int pg_gpu_stage_async(pg_ctx* c, const float* base, const uint32_t* gt,
   PG_HIP(c, hipMemcpyAsync(c->h_gt_dev, gt,
                            uint64_t(p.n_query) * p.gt_k * sizeof(uint32_t),
                            hipMemcpyHostToDevice, c->stream));
-  PG_TRY(run_pipeline(c, c->h_base_dev, c->h_gt_dev, c->h_proj_dev,
-                      c->h_link_dev));
+  if (c->p.tune_flags & kTuneGraph)
+    PG_TRY(run_pipeline_graph(c, c->h_base_dev, c->h_gt_dev, c->h_proj_dev,
+                              c->h_link_dev));
+  else
+    PG_TRY(run_pipeline(c, c->h_base_dev, c->h_gt_dev, c->h_proj_dev,
+                        c->h_link_dev));
   PG_HIP(c, hipMemcpyAsync(out_proj, c->h_proj_dev,
                            N * p.proj_deg * sizeof(uint32_t),
                            hipMemcpyDeviceToHost, c->stream));
@@ -596,7 +660,10 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
                               d_graph, graph_deg, p.Km, p.Kp, p.Kd, p.topk,
                               p.reset_iter, p.entry_seed, p.max_iters,
                               effective_table_size(p), tune_gather(p), d_out,
-                              c->stats, c->stream));
+                              c->stats,
+                              (p.tune_flags & kTunePersistent) ? c->queue
+                                                               : nullptr,
+                              c->stream));
   return PG_OK;
 }
 

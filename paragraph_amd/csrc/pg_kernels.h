@@ -44,7 +44,8 @@ hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
                                  uint32_t dpad, void* out16, float* norms,
                                  hipStream_t s);
 // base16 == nullptr: exact FP32 search; else BF16-screened search over the
-// padded BF16 copy (row stride dpad elements).
+// padded BF16 copy (row stride dpad elements).  queue != nullptr: persistent
+// grid fed by the device counter *queue (reset by the launcher).
 hipError_t launch_search(const float* queries, uint32_t n_query,
                          const float* base, const void* base16, uint32_t dpad,
                          uint32_t n_base, uint32_t dim,
@@ -53,6 +54,7 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t reset_iter, uint32_t entry_seed,
                          uint32_t max_iters, uint32_t table_size,
                          uint32_t gather_batch, uint32_t* out,
-                         unsigned long long* stats, hipStream_t s);
+                         unsigned long long* stats, uint32_t* queue,
+                         hipStream_t s);
 
 }  // namespace pg

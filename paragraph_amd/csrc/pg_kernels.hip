@@ -856,7 +856,7 @@ __global__ __launch_bounds__(64) void search_kernel(
     uint32_t Kp, uint32_t Kd, uint32_t topk, uint32_t reset_iter,
     uint32_t entry_seed, uint32_t max_iters, uint32_t tsize, uint32_t lp2,
     uint32_t kd_shift, uint32_t* __restrict__ out,
-    unsigned long long* __restrict__ stats) {
+    unsigned long long* __restrict__ stats, uint32_t* __restrict__ queue) {
   extern __shared__ __align__(16) uint8_t smem[];
   const uint32_t kpkd = Kp * Kd;
   ev.qs = reinterpret_cast<bf16x8*>(smem);
@@ -868,8 +868,16 @@ __global__ __launch_bounds__(64) void search_kernel(
   uint32_t* explore = todo + kpkd;
   // sorted misses of the merge step: reuses cand + todo (2*kpkd words)
   uint64_t* ms = reinterpret_cast<uint64_t*>(cand);
-  const uint32_t q = blockIdx.x;
   const uint32_t lane = lane_id();
+  // One query per workgroup, or (queue != nullptr) a persistent grid whose
+  // waves take query ids from a device counter until it runs past n_query.
+  auto next_query = [&]() -> uint32_t {
+    uint32_t v = 0;
+    if (lane == 0) v = atomicAdd(queue, 1u);
+    return __shfl(v, 0, 64);
+  };
+  uint32_t q = queue ? next_query() : blockIdx.x;
+  while (q < n_query) {
   const bool fast_merge = Km <= 64u * W && kpkd <= 64u * W;
   ev.load_query(queries, q);
   // initial list: [0, Km) = (FLT_MAX, tomb); [Km, Km+kpkd) = entry points
@@ -988,6 +996,10 @@ __global__ __launch_bounds__(64) void search_kernel(
     stat_add(stats, ST_SEARCH_ITERS, total_iter);
     if (capped) stat_add(stats, ST_SEARCH_CAPPED, 1);
   }
+  if (queue == nullptr) break;
+  __syncthreads();
+  q = next_query();
+  }  // query loop
 }
 
 // ------------------------------------------------------------ launchers
@@ -1138,7 +1150,7 @@ static hipError_t launch_search_eval(
     const uint32_t* graph, uint32_t gdeg, uint32_t Km, uint32_t Kp,
     uint32_t Kd, uint32_t topk, uint32_t reset_iter, uint32_t entry_seed,
     uint32_t max_iters, uint32_t table_size, uint32_t* out,
-    unsigned long long* stats, hipStream_t s) {
+    unsigned long long* stats, uint32_t* queue, hipStream_t s) {
   uint32_t lp2 = 64;
   while (lp2 < Km + Kp * Kd) lp2 <<= 1;
   const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp) +
@@ -1146,18 +1158,27 @@ static hipError_t launch_search_eval(
   uint32_t kd_shift = 0xFFu;
   for (uint32_t sh = 0; sh < 31; ++sh)
     if ((1u << sh) == Kd) kd_shift = sh;
-#define PG_SEARCH_W(WW)                                                     \
-  do {                                                                      \
-    (void)hipFuncSetAttribute(                                              \
-        reinterpret_cast<const void*>(&search_kernel<Eval, WW>),            \
-        hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));             \
-    search_kernel<Eval, WW><<<n_query, 64, smem, s>>>(                      \
-        ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk,        \
-        reset_iter, entry_seed, max_iters, table_size, lp2, kd_shift, out,  \
-        stats);                                                             \
-  } while (0)
-  PG_SEARCH_W(4);
-#undef PG_SEARCH_W
+  const void* fn = reinterpret_cast<const void*>(&search_kernel<Eval, 4>);
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(smem));
+  uint32_t grid = n_query;
+  if (queue) {
+    // persistent grid: as many one-wave workgroups as can be resident
+    int dev = 0, per_cu = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, smem);
+    if (per_cu < 1) per_cu = 1;
+    if (cus < 1) cus = 1;
+    const uint64_t resident = uint64_t(per_cu) * uint64_t(cus);
+    grid = uint32_t(resident < n_query ? resident : n_query);
+    hipError_t e = launch_fill_u32(queue, 1, 0u, s);
+    if (e != hipSuccess) return e;
+  }
+  search_kernel<Eval, 4><<<grid, 64, smem, s>>>(
+      ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,
+      entry_seed, max_iters, table_size, lp2, kd_shift, out, stats, queue);
   return hipGetLastError();
 }
 
@@ -1169,13 +1190,14 @@ static hipError_t launch_f32(const float* base, uint32_t dim,
                              uint32_t Kd, uint32_t topk, uint32_t reset_iter,
                              uint32_t entry_seed, uint32_t max_iters,
                              uint32_t table_size, uint32_t* out,
-                             unsigned long long* stats, hipStream_t s) {
+                             unsigned long long* stats, uint32_t* queue,
+                             hipStream_t s) {
   EvalF32<TQ, U> ev;
   ev.base = base;
   ev.dim = dim;
   return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
                             Kd, topk, reset_iter, entry_seed, max_iters,
-                            table_size, out, stats, s);
+                            table_size, out, stats, queue, s);
 }
 
 template <int NCH, int G>
@@ -1186,7 +1208,8 @@ static hipError_t launch_bf16(const void* base16, uint32_t dim, uint32_t dpad,
                               uint32_t Kd, uint32_t topk, uint32_t reset_iter,
                               uint32_t entry_seed, uint32_t max_iters,
                               uint32_t table_size, uint32_t* out,
-                              unsigned long long* stats, hipStream_t s) {
+                              unsigned long long* stats, uint32_t* queue,
+                              hipStream_t s) {
   EvalBF16<NCH, G> ev;
   ev.base16 = reinterpret_cast<const __bf16*>(base16);
   ev.dim = dim;
@@ -1194,7 +1217,7 @@ static hipError_t launch_bf16(const void* base16, uint32_t dim, uint32_t dpad,
   ev.nch = dpad / 32;
   return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
                             Kd, topk, reset_iter, entry_seed, max_iters,
-                            table_size, out, stats, s);
+                            table_size, out, stats, queue, s);
 }
 
 hipError_t launch_search(const float* queries, uint32_t n_query,
@@ -1205,11 +1228,12 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t reset_iter, uint32_t entry_seed,
                          uint32_t max_iters, uint32_t table_size,
                          uint32_t gather_batch, uint32_t* out,
-                         unsigned long long* stats, hipStream_t s) {
+                         unsigned long long* stats, uint32_t* queue,
+                         hipStream_t s) {
   if (n_query == 0) return hipSuccess;
 #define PG_ARGS                                                             \
   queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,      \
-      entry_seed, max_iters, table_size, out, stats, s
+      entry_seed, max_iters, table_size, out, stats, queue, s
   if (base16 != nullptr) {
     // screened path: gather_batch = groups of 16 candidates in flight
     // NCH chunks of 32 (8 or 16; chunks past dpad/32 are zero and skip

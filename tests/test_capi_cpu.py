@@ -35,6 +35,7 @@ def test_params_init_mapping():
     assert (p.proj_deg, p.proj_pr, p.proj_rev) == (16, 15, 111)
     assert (p.reset_iter, p.table_size, p.rounds, p.entry_seed) == (
         15, 0, 2, 3030517)
+    assert p.tune_flags == 0x20  # persistent search grid
     p = _native.make_params(100_000, 200, 100_000, 32, 100)
     assert (p.Kp, p.topk) == (7, 200)
     assert _native.check_params(p)[0] == 0

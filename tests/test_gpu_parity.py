@@ -221,3 +221,56 @@ def test_build_is_deterministic(cuda):
             outs.append((from_dev(proj_d), from_dev(link_d)))
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_persistent_search_matches_oracle(cuda):
+    """The persistent work-queue grid (tune bit 0x20) computes the same rows
+    as one workgroup per query."""
+    import torch
+    w = workload("d200_m32_l100")
+    n = w["params"].n_base
+    rng = np.random.default_rng(5)
+    graph = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    base_d, graph_d = to_dev(w["base"], torch), to_dev(graph, torch)
+    p = _native.make_params(n, w["base"].shape[1], n, 32, 100,
+                            tune_flags=0x20)
+    out_d = torch.empty((n, p.topk), dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.run_search(base_d, base_d, graph_d, out_d)
+        ctx.sync()
+    ref = cpu_ref.search(w["base"], w["base"], graph, p.Km, p.Kp, p.Kd,
+                         p.topk)
+    np.testing.assert_array_equal(from_dev(out_d), ref)
+
+
+@pytest.mark.parametrize("prec", [_native.PG_FP32, _native.PG_BF16])
+def test_graph_replays_match_eager(prec, cuda):
+    """hipGraph mode (tune bit 0x40): every replay, with eager work and
+    synchronisations in between, equals the eager build."""
+    import torch
+    w = workload("d200_m32_l100")
+    p0 = w["params"]
+    n = p0.n_base
+    base_d, gt_d = to_dev(w["base"], torch), to_dev(w["gt_u32"], torch)
+    kw = dict(precision=prec)
+    pe = _native.make_params(n, p0.dim, n, p0.M, p0.Km, **kw)
+    pg = _native.make_params(n, p0.dim, n, p0.M, p0.Km,
+                             tune_flags=0x40 | 0x20, **kw)
+    proj_e = torch.empty((n, 16), dtype=torch.int32, device=cuda)
+    link_e = torch.empty((n, p0.M), dtype=torch.int32, device=cuda)
+    with _native.Context(pe) as ctx:
+        ctx.stage_device(base_d, gt_d, proj_e, link_e)
+        ctx.wait()
+    ref = (from_dev(proj_e), from_dev(link_e))
+    proj_g = torch.empty_like(proj_e)
+    link_g = torch.empty_like(link_e)
+    with _native.Context(pg) as ctx:
+        for r in range(3):
+            proj_g.fill_(-7)
+            link_g.fill_(-7)
+            torch.cuda.synchronize()
+            ctx.stage_device(base_d, gt_d, proj_g, link_g)
+            ctx.wait()
+            np.testing.assert_array_equal(from_dev(proj_g), ref[0])
+            np.testing.assert_array_equal(from_dev(link_g), ref[1])
+            _ = (base_d * 2.0).sum().item()  # eager work between replays
