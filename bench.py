@@ -138,10 +138,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="cfg2_1m_d200")
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--precision", default="both",
-                    choices=["fp32", "bf16", "both"],
-                    help="both: time the plain FP32 path and the BF16-"
-                         "screened path; value = the screened path")
+    ap.add_argument("--precision", default="fp32,bf16",
+                    help="comma list of builds to time (fp32, bf16, fp8); "
+                         "value = the first screened one (bf16 / fp8), or "
+                         "fp32 when it is the only one")
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
@@ -187,7 +187,12 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    precs = ["fp32", "bf16"] if args.precision == "both" else [args.precision]
+    precs = [x.strip() for x in args.precision.split(",") if x.strip()]
+    if args.precision == "both":
+        precs = ["fp32", "bf16"]
+    for x in precs:
+        if x not in ("fp32", "bf16", "fp8"):
+            ap.error(f"unknown precision {x}")
     results = {}
     tf32 = {} if args.tune_flags is None else {"tune_flags": args.tune_flags}
     tf16 = {} if args.tune_flags_bf16 is None else {
@@ -214,8 +219,9 @@ def main():
                       if kv)
             kw = dict(tf16)
             kw.update({k: int(v) for k, v in ov.items()})
-            p = _native.make_params(n, dim, n, M, L,
-                                    precision=_native.PG_BF16, **kw)
+            p = _native.make_params(
+                n, dim, n, M, L, precision=_native.PG_BF16 if prec == "bf16"
+                else _native.PG_FP8E4M3, **kw)
         proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
         link_d = torch.empty((n, M), dtype=torch.int32, device=dev)
         ctx = _native.Context(p)
@@ -257,22 +263,26 @@ def main():
         cpu = cpu_baseline(ctx32, p32, w, gt_u32, base_d, gt_d,
                            args.cpu_sample, log)
 
-    head = results["bf16"] if "bf16" in results else results["fp32"]
-    head_prec = "bf16" if "bf16" in results else "fp32"
+    screened = [x for x in precs if x != "fp32"]
+    head_prec = screened[0] if screened else "fp32"
+    head = results[head_prec]
     extra = {}
     for prec, r in results.items():
         extra[prec] = {k: (round(v, 3) if isinstance(v, float) else v)
                        for k, v in r.items() if k not in ("link", "ctx")}
-    if "fp32" in results and "bf16" in results:
-        extra["speedup_bf16_vs_fp32"] = round(
-            results["fp32"]["ms_per_step"] / results["bf16"]["ms_per_step"], 3)
+    for sp in screened:
+        if "fp32" not in results:
+            break
+        extra[f"speedup_{sp}_vs_fp32"] = round(
+            results["fp32"]["ms_per_step"] / results[sp]["ms_per_step"], 3)
         if truth is not None:
-            extra["recall_delta_pt"] = round(
-                100 * (results["bf16"]["recall_at_10"] -
+            extra[f"recall_delta_pt_{sp}"] = round(
+                100 * (results[sp]["recall_at_10"] -
                        results["fp32"]["recall_at_10"]), 2)
         a = results["fp32"]["link"].cpu().numpy()
-        b = results["bf16"]["link"].cpu().numpy()
-        extra["link_overlap_bf16_vs_fp32"] = round(float(np.mean([
+        b = results[sp]["link"].cpu().numpy()
+        # per-node neighbour-set overlap of the link rows (first 20k nodes)
+        extra[f"link_overlap_{sp}_vs_fp32"] = round(float(np.mean([
             len(set(x[x >= 0]) & set(y[y >= 0])) / max(1, len(set(x[x >= 0])))
             for x, y in zip(a[:20000], b[:20000])])), 4)
     line = {
@@ -287,14 +297,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": head_prec,
+        "dtype": {"fp32": "f32", "bf16": "bf16", "fp8": "fp8e4m3"}[head_prec],
         "data": "synthetic (seeded 64-component Gaussian mixture; exact "
                 "top-128 inner-product gt)",
         "config": {"workload": f"{args.config}: {n} base x {dim} f32, {n} "
                    f"training queries, gt k=128, M={M}, L={L}, inner "
                    f"product; value = {head_prec} path"
-                   + (" (BF16 MFMA search screening, FP32 re-rank and "
-                      "prune)" if head_prec == "bf16" else ""),
+                   + ({"bf16": " (BF16 MFMA search screening, FP32 "
+                       "re-rank, BF16-screened exact prune)",
+                       "fp8": " (FP8 e4m3 per-vector-scaled MFMA search "
+                       "screening, FP32 re-rank, BF16-screened exact "
+                       "prune)"}.get(head_prec, "")),
                    "n_base": n, "dim": dim, "M": M, "L": L,
                    "precision": head_prec},
         "roofline": head["roofline"],

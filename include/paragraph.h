@@ -79,7 +79,9 @@ typedef struct pg_params {
                            persistent search grid fed by a device work
                            counter; bit 6 (0x40): capture the pipeline in a
                            hipGraph once per buffer set and replay it (no
-                           per-stage times in that mode) */
+                           per-stage times in that mode); bit 7 (0x80):
+                           split-BF16 screen in the screened prune / merge
+                           (tighter bound, fewer exact fallbacks) */
   uint32_t reserved[7];
 } pg_params;
 

@@ -51,9 +51,12 @@ struct pg_ctx {
   uint32_t* srcs = nullptr;
   uint32_t* res = nullptr;
   void* base16 = nullptr;   // screened paths: BF16 copy of the base
+  void* base16lo = nullptr; // split screen: residual BF16 copy
+  void* base8 = nullptr;    // FP8 path: per-row scaled e4m3 rows
+  float* scale8 = nullptr;  // FP8 path: row scales
   float* norms = nullptr;   // screened paths: FP32 row norms
   uint32_t dpad = 0;        // BF16 row stride (dim rounded up to 32)
-  float eps_c = 0.0f;       // screening bound factor
+  float eps_c = 0.0f;       // screening bound factor (plain or split)
   unsigned long long* stats = nullptr;
   unsigned long long* stage_cnt = nullptr;  // (kMaxStages+1) x 16 snapshots
   uint32_t* queue = nullptr;                // persistent-search work counter
@@ -129,8 +132,11 @@ uint32_t effective_table_size(const pg_params& p) {
 //             BF16: 16-candidate groups in flight, 1/2)
 //   bit 5     persistent search grid fed by a device work counter
 //   bit 6     capture the pipeline in a hipGraph once, replay it per build
+//   bit 7     split-BF16 screen in prune / merge (x = hi + lo, tighter
+//             bound, twice the screen bytes)
 constexpr uint32_t kTunePersistent = 0x20u;
 constexpr uint32_t kTuneGraph = 0x40u;
+constexpr uint32_t kTuneSplit = 0x80u;
 uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
 
 // Bound factor of the BF16 screen against the exact FP32 chain:
@@ -142,12 +148,30 @@ uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
 //   (1 + u)^2 sum|x y|),
 //   the chain's own FP32 error (gamma over dim terms, u = 2^-24),
 // inflated by 1% for the FP32 norms and the comparisons themselves.
+float screen_eps_split(uint32_t dim, uint32_t dpad);
+
 float screen_eps(uint32_t dim, uint32_t dpad) {
   const double u = 1.0 / 256.0;
   const double ua = std::ldexp(1.0, -23);
   const double ga = dpad * ua / (1.0 - dpad * ua);
   const double gc = dim * std::ldexp(1.0, -24) / (1.0 - dim * std::ldexp(1.0, -24));
   const double e = (2 * u + u * u) + ga * (1 + u) * (1 + u) + gc;
+  return float(e * 1.01);
+}
+
+// Split screen: x = hi + lo + r with |hi| <= (1+u)|x|, |lo| <= u(1+u)|x|,
+// |r| <= u^2|x| (u = 2^-8).  The three products hi.hi, lo.hi, hi.lo are
+// exact in FP32; the dropped terms hi.r', lo.lo', lo.r', r.y are below
+// (3u^2 + 4u^3 + 2u^4)|x y|.  The MFMA accumulates 3*dpad terms whose
+// magnitudes sum to at most (1+u)^2 (1+2u) sum|x y| (gamma with 2^-23); the
+// chain's own error is gamma(dim) with 2^-24.  Inflated by 1%.
+float screen_eps_split(uint32_t dim, uint32_t dpad) {
+  const double u = 1.0 / 256.0;
+  const double na = 3.0 * dpad, ua = std::ldexp(1.0, -23);
+  const double ga = na * ua / (1.0 - na * ua);
+  const double gc = dim * std::ldexp(1.0, -24) / (1.0 - dim * std::ldexp(1.0, -24));
+  const double e = 3 * u * u + 4 * u * u * u + 2 * u * u * u * u +
+                   ga * (1 + u) * (1 + u) * (1 + 2 * u) + gc;
   return float(e * 1.01);
 }
 
@@ -174,13 +198,18 @@ int record(pg_ctx* c, int idx) {
   } while (0)
 
 pg::ScreenArgs screen_args(const pg_ctx* c) {
-  return pg::ScreenArgs{reinterpret_cast<const __bf16*>(c->base16), c->norms,
+  return pg::ScreenArgs{reinterpret_cast<const __bf16*>(c->base16),
+                        reinterpret_cast<const __bf16*>(c->base16lo), c->norms,
                         c->dpad, c->eps_c};
 }
 
 int prepare_screen(pg_ctx* c, const float* d_base) {
   PG_HIP(c, pg::launch_prepare_screen(d_base, c->p.n_base, c->p.dim, c->dpad,
-                                      c->base16, c->norms, c->stream));
+                                      c->base16, c->base16lo, c->norms,
+                                      c->stream));
+  if (c->base8)
+    PG_HIP(c, pg::launch_prepare_fp8(d_base, c->p.n_base, c->p.dim, c->dpad,
+                                     c->base8, c->scale8, c->stream));
   return PG_OK;
 }
 
@@ -211,7 +240,9 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   const uint32_t* graph = d_proj;
   uint32_t gdeg = p.proj_deg;
   for (uint32_t r = 0; r < p.rounds; ++r) {
-    PG_HIP(c, pg::launch_search(d_base, N, d_base, c->base16, c->dpad, N,
+    PG_HIP(c, pg::launch_search(d_base, N, d_base,
+                                c->base8 ? nullptr : c->base16, c->base8,
+                                c->scale8, c->dpad, N,
                                 p.dim, graph, gdeg, p.Km,
                                 p.Kp, p.Kd, p.topk, p.reset_iter,
                                 p.entry_seed, p.max_iters,
@@ -358,10 +389,11 @@ int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len) {
   }
   if (p->rounds > 3) return fail("rounds must be <= 3");
   if (p->max_iters == 0) return fail("max_iters must be > 0");
-  if (p->precision != PG_FP32 && p->precision != PG_BF16)
-    return fail("precision must be PG_FP32 or PG_BF16 in this build");
-  if (p->precision == PG_BF16 && p->dim > 512)
-    return fail("the BF16-screened path supports dim <= 512");
+  if (p->precision != PG_FP32 && p->precision != PG_BF16 &&
+      p->precision != PG_FP8E4M3)
+    return fail("precision must be PG_FP32, PG_BF16 or PG_FP8E4M3");
+  if (p->precision != PG_FP32 && p->dim > 512)
+    return fail("the screened paths support dim <= 512");
 
   if (msg && msg_len) msg[0] = 0;
   return PG_OK;
@@ -392,7 +424,8 @@ pg_ctx* pg_create(int device, const pg_params* p) {
   c->p = *p;
   c->mp_max = p->proj_pr > p->M ? p->proj_pr : p->M;
   c->dpad = (p->dim + 31) / 32 * 32;
-  c->eps_c = screen_eps(p->dim, c->dpad);
+  c->eps_c = (p->tune_flags & kTuneSplit) ? screen_eps_split(p->dim, c->dpad)
+                                           : screen_eps(p->dim, c->dpad);
   auto fail = [&](int) {
     std::memcpy(g_err, c->err, sizeof(g_err));
     pg_destroy(c);
@@ -422,10 +455,17 @@ pg_ctx* pg_create(int device, const pg_params* p) {
       dev_alloc(c, &c->scan_tmp, nb, "scan_tmp") ||
       dev_alloc(c, &c->srcs, N * c->mp_max, "srcs") ||
       dev_alloc(c, &c->res, N * p->topk, "search_out") ||
-      (p->precision == PG_BF16 &&
+      (p->precision != PG_FP32 &&
        (dev_alloc(c, reinterpret_cast<uint16_t**>(&c->base16),
                   N * ((p->dim + 31) / 32 * 32), "base_bf16") ||
         dev_alloc(c, &c->norms, N, "norms"))) ||
+      (p->precision != PG_FP32 && (p->tune_flags & kTuneSplit) &&
+       dev_alloc(c, reinterpret_cast<uint16_t**>(&c->base16lo),
+                 N * ((p->dim + 31) / 32 * 32), "base_bf16_lo")) ||
+      (p->precision == PG_FP8E4M3 &&
+       (dev_alloc(c, reinterpret_cast<uint8_t**>(&c->base8),
+                  N * ((p->dim + 31) / 32 * 32), "base_fp8") ||
+        dev_alloc(c, &c->scale8, N, "scale8"))) ||
       dev_alloc(c, &c->stats, kStatCount, "stats") ||
       dev_alloc(c, &c->stage_cnt, (kMaxStages + 1) * 16ull, "stage_cnt") ||
       dev_alloc(c, &c->queue, 1, "queue"))
@@ -452,7 +492,10 @@ void pg_destroy(pg_ctx* c) {
   dev_free(c->srcs);
   dev_free(c->res);
   if (c->base16) (void)hipFree(c->base16);
-  c->base16 = nullptr;
+  if (c->base8) (void)hipFree(c->base8);
+  if (c->base16lo) (void)hipFree(c->base16lo);
+  c->base16 = c->base8 = c->base16lo = nullptr;
+  dev_free(c->scale8);
   dev_free(c->norms);
   drop_graph(c);
   dev_free(c->stats);
@@ -655,7 +698,9 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
                    graph_deg);
   PG_HIP(c, hipSetDevice(c->device));
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
-  PG_HIP(c, pg::launch_search(d_queries, n_query, d_base, c->base16, c->dpad,
+  PG_HIP(c, pg::launch_search(d_queries, n_query, d_base,
+                              c->base8 ? nullptr : c->base16, c->base8,
+                              c->scale8, c->dpad,
                               p.n_base, p.dim,
                               d_graph, graph_deg, p.Km, p.Kp, p.Kd, p.topk,
                               p.reset_iter, p.entry_seed, p.max_iters,

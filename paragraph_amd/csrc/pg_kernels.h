@@ -9,6 +9,7 @@ namespace pg {
 // Screened-path inputs of the prune / merge kernels (see pg_kernels.hip).
 struct ScreenArgs {
   const __bf16* base16;    // bf16(x), rows padded to dpad, zero-filled
+  const __bf16* base16lo;  // bf16(x - hi) for the split screen, or nullptr
   const float* norms;      // FP32 row norms
   uint32_t dpad;
   float eps_c;             // |screened - exact chain| <= eps_c * |x| * |y|
@@ -41,13 +42,18 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
 size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
                          uint32_t Kp);
 hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
-                                 uint32_t dpad, void* out16, float* norms,
-                                 hipStream_t s);
+                                 uint32_t dpad, void* out16, void* lo16,
+                                 float* norms, hipStream_t s);
 // base16 == nullptr: exact FP32 search; else BF16-screened search over the
 // padded BF16 copy (row stride dpad elements).  queue != nullptr: persistent
 // grid fed by the device counter *queue (reset by the launcher).
+hipError_t launch_prepare_fp8(const float* base, uint32_t n, uint32_t dim,
+                              uint32_t dpad, void* out8, float* scale,
+                              hipStream_t s);
+// base8 != nullptr: FP8-screened search over per-row-scaled e4m3 rows.
 hipError_t launch_search(const float* queries, uint32_t n_query,
-                         const float* base, const void* base16, uint32_t dpad,
+                         const float* base, const void* base16,
+                         const void* base8, const float* scale8, uint32_t dpad,
                          uint32_t n_base, uint32_t dim,
                          const uint32_t* graph, uint32_t gdeg, uint32_t Km,
                          uint32_t Kp, uint32_t Kd, uint32_t topk,

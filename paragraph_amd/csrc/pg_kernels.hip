@@ -146,10 +146,11 @@ __device__ uint32_t rng_select(const float* __restrict__ base, uint32_t n_base,
 // pair distances taken from BF16 MFMA Gram tiles.  Candidates are resolved
 // in blocks of 16 (lane l%16 = block candidate): v_mfma_f32_16x16x32_bf16
 // chains give the block against each 16-wide tile of kept entries and
-// against the block itself (BF16 rows, FP32 accumulation).  A screened
-// value s decides a comparison only when the bound |s - chain| <=
-// eps_c*|c|*|k| (BF16 rounding of both rows, FP32 accumulation, the chain's
-// own error, Cauchy-Schwarz) puts both sides of d_i out of reach;
+// against the block itself (BF16 rows, or the hi + lo split of each row for a
+// ~30x tighter bound; FP32 accumulation).  A screened value s decides a
+// comparison only when the bound |s - chain| <= eps_c*|c|*|k| (rounding of
+// the rows, FP32 accumulation, the chain's own error, Cauchy-Schwarz) puts
+// both sides of d_i out of reach;
 // otherwise the exact FP32 chain is computed (counted as a fallback).  So
 // every decision equals the plain FP32 path's.
 // 1: occluded for sure, 0: not occluded for sure, -1: undecided.
@@ -159,7 +160,7 @@ __device__ __forceinline__ int screen_cmp(float s, float d, float e) {
   return -1;
 }
 
-template <bool kStopIds, int NCH, int NKT>
+template <bool kStopIds, int NCH, int NKT, bool kSplit>
 __device__ uint32_t rng_select_screened(
     const float* __restrict__ base, const ScreenArgs& sc, uint32_t n_base,
     uint32_t dim, const uint64_t* skey, const uint32_t* sid, uint32_t n_scan,
@@ -192,17 +193,30 @@ __device__ uint32_t rng_select_screened(
     const bool active = in && m < lim;
     const uint32_t nk0 = nk;
     // ---- Gram tiles
-    bf16x8 a[NCH];
+    // Gram tiles.  Plain: BF16 rows.  Split (kSplit): rows x = hi + lo as two
+    // BF16 copies and x.y taken as hi.hi + lo.hi + hi.lo (three MFMAs).
+    bf16x8 ah[NCH], al[kSplit ? NCH : 1];
     {
-      const bf16x8* arow = reinterpret_cast<const bf16x8*>(
-          sc.base16 + size_t(active ? my_id : 0u) * sc.dpad + 8u * kq);
+      const size_t off = size_t(active ? my_id : 0u) * sc.dpad + 8u * kq;
+      const bf16x8* rh = reinterpret_cast<const bf16x8*>(sc.base16 + off);
+      const bf16x8* rl = reinterpret_cast<const bf16x8*>(
+          kSplit ? sc.base16lo + off : sc.base16 + off);
 #pragma unroll
       for (int c = 0; c < NCH; ++c) {
-        if (active && uint32_t(c) < nch) {
-          a[c] = arow[4 * c];
+        const bool ld = active && uint32_t(c) < nch;
+        if (ld) {
+          ah[c] = rh[4 * c];
         } else {
 #pragma unroll
-          for (int j = 0; j < 8; ++j) a[c][j] = (__bf16)0.0f;
+          for (int j = 0; j < 8; ++j) ah[c][j] = (__bf16)0.0f;
+        }
+        if constexpr (kSplit) {
+          if (ld) {
+            al[c] = rl[4 * c];
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) al[c][j] = (__bf16)0.0f;
+          }
         }
       }
     }
@@ -211,19 +225,33 @@ __device__ uint32_t rng_select_screened(
       if (16u * t < nk0) {
         const uint32_t kidx = 16u * t + m;
         const bool kv = kidx < nk0;
-        const bf16x8* brow = reinterpret_cast<const bf16x8*>(
-            sc.base16 + size_t(kv ? kept[kidx] : 0u) * sc.dpad + 8u * kq);
+        const size_t off = size_t(kv ? kept[kidx] : 0u) * sc.dpad + 8u * kq;
+        const bf16x8* bh = reinterpret_cast<const bf16x8*>(sc.base16 + off);
+        const bf16x8* bl = reinterpret_cast<const bf16x8*>(
+            kSplit ? sc.base16lo + off : sc.base16 + off);
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-          bf16x8 b;
-          if (kv && uint32_t(c) < nch) {
-            b = brow[4 * c];
+          const bool ld = kv && uint32_t(c) < nch;
+          bf16x8 h;
+          if (ld) {
+            h = bh[4 * c];
           } else {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) b[j] = (__bf16)0.0f;
+            for (int j = 0; j < 8; ++j) h[j] = (__bf16)0.0f;
           }
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c], b, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], h, acc, 0, 0, 0);
+          if constexpr (kSplit) {
+            bf16x8 lw;
+            if (ld) {
+              lw = bl[4 * c];
+            } else {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) lw[j] = (__bf16)0.0f;
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], h, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], lw, acc, 0, 0, 0);
+          }
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) gs[(4u * kq + j) * GW + 16u * t + m] = acc[j];
@@ -232,8 +260,13 @@ __device__ uint32_t rng_select_screened(
     {
       f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int c = 0; c < NCH; ++c)
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c], a[c], acc, 0, 0, 0);
+      for (int c = 0; c < NCH; ++c) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], ah[c], acc, 0, 0, 0);
+        if constexpr (kSplit) {
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], ah[c], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], al[c], acc, 0, 0, 0);
+        }
+      }
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         gs[(4u * kq + j) * GW + NKT * 16u + m] = acc[j];
@@ -308,7 +341,7 @@ __device__ uint32_t rng_select_screened(
 // from the pruned lists (reverse_* kernels).
 // NCH > 0 selects the screened RNG (BF16 MFMA tiles, NCH 32-wide chunks,
 // NKT 16-wide kept tiles); NCH == 0 is the plain FP32 path.
-template <uint32_t DP2, int NCH, int NKT>
+template <uint32_t DP2, int NCH, int NKT, bool kSplit>
 __global__ __launch_bounds__(64) void prune_kernel(
     const float* __restrict__ base, uint32_t n_base, uint32_t dim,
     const uint32_t* __restrict__ rows, uint32_t D,
@@ -368,7 +401,7 @@ __global__ __launch_bounds__(64) void prune_kernel(
       nk = rng_select<true>(base, n_base, dim, skey, sid, D, mp, kept, 1,
                             &pairs);
     else
-      nk = rng_select_screened<true, NCH, NKT>(base, sc, n_base, dim, skey,
+      nk = rng_select_screened<true, NCH, NKT, kSplit>(base, sc, n_base, dim, skey,
                                                sid, D, mp, kept, kept_norm, 1,
                                                gs, &pairs, &fbk);
   }
@@ -503,7 +536,7 @@ __global__ void scan_add_kernel(uint32_t* __restrict__ out, uint32_t n,
 // (Gbuilder.cuh:1559-1594), tomb fill.
 constexpr uint32_t kSegChunk = 512;
 
-template <int NCH, int NKT>
+template <int NCH, int NKT, bool kSplit>
 __global__ __launch_bounds__(64) void merge_kernel(
     const float* __restrict__ base, uint32_t n_base, uint32_t dim,
     const uint32_t* __restrict__ off, const uint32_t* __restrict__ srcs,
@@ -574,7 +607,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
     nk = rng_select<false>(base, n_base, dim, skey, sid, n, deg, kept, 1,
                            &pairs);
   else
-    nk = rng_select_screened<false, NCH, NKT>(base, sc, n_base, dim, skey, sid,
+    nk = rng_select_screened<false, NCH, NKT, kSplit>(base, sc, n_base, dim, skey, sid,
                                               n, deg, kept, kept_norm, 1, gs,
                                               &pairs, &fbk);
   if (!strict && nk < deg) {
@@ -777,6 +810,94 @@ struct EvalBF16 {
           if (idx < ntodo) {
             const uint32_t t = todo[idx];
             tail[t] = search_key(-dot, cand[t]);
+          }
+        }
+      }
+    }
+  }
+};
+
+// EvalFP8: screening distances on MFMA in FP8 e4m3 (OCP e4m3fn on gfx950,
+// max 448) with one scale per vector: row x is stored as e4m3(x / s_x),
+// s_x = max|x| / 448, and the query likewise; v_mfma_f32_16x16x32_fp8_fp8
+// accumulates in FP32 and the distance is -(s_q s_x) * acc.  Same operand
+// layout as EvalBF16 (lane l: row l%16, bytes 32c + 8(l/16) .. +7).
+__device__ __forceinline__ long pack_fp8x8(const float* v) {
+  int lo = 0, hi = 0;
+  lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], lo, false);
+  lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], lo, true);
+  hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4], v[5], hi, false);
+  hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6], v[7], hi, true);
+  return long(uint32_t(lo)) | (long(uint32_t(hi)) << 32);
+}
+
+template <int NCH, int G>
+struct EvalFP8 {
+  static constexpr bool kExactVisited = false;
+  static constexpr uint32_t kQueryLdsBytes = 0;
+  bf16x8* qs;  // unused
+  long qf[NCH];
+  float qscale;
+  const uint8_t* __restrict__ base8;
+  const float* __restrict__ scale8;
+  uint32_t dim, dpad, nch;
+
+  __device__ __forceinline__ void load_query(const float* __restrict__ queries,
+                                             uint32_t q) {
+    const uint32_t lane = lane_id();
+    const float* x = queries + size_t(q) * dim;
+    float mx = 0.0f;
+    for (uint32_t e = lane; e < dim; e += kWave) mx = fmaxf(mx, fabsf(x[e]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    qscale = mx > 0.0f ? mx / 448.0f : 1.0f;
+    const float inv = 1.0f / qscale;
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const uint32_t e = 32u * c + 8u * (lane >> 4) + j;
+        v[j] = (uint32_t(c) < nch && e < dim) ? x[e] * inv : 0.0f;
+      }
+      qf[c] = pack_fp8x8(v);
+    }
+  }
+
+  __device__ __forceinline__ void eval(const uint32_t* cand,
+                                       const uint32_t* todo, uint32_t ntodo,
+                                       uint64_t* tail) const {
+    const uint32_t lane = lane_id();
+    const uint32_t m = lane & 15u, kq = lane >> 4;
+    for (uint32_t g0 = 0; g0 < ntodo; g0 += 16u * G) {
+      long a[G][NCH];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const uint32_t idx = g0 + 16u * g + m;
+        const bool ok = idx < ntodo;
+        const uint32_t cid = ok ? cand[todo[idx]] : 0u;
+        const long* row = reinterpret_cast<const long*>(
+            base8 + size_t(cid) * dpad + 8u * kq);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+          a[g][c] = (ok && uint32_t(c) < nch) ? row[4 * c] : 0l;
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < NCH; ++c)
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[g][c], qf[c], acc,
+                                                           0, 0, 0);
+        if (m < 4u) {
+          const uint32_t cm = 4u * kq + m;
+          const uint32_t idx = g0 + 16u * g + cm;
+          const float dot = m == 0 ? acc[0] : m == 1 ? acc[1]
+                          : m == 2 ? acc[2] : acc[3];
+          if (idx < ntodo) {
+            const uint32_t t = todo[idx];
+            const uint32_t cid = cand[t];
+            tail[t] = search_key(-(dot * qscale * scale8[cid]), cid);
           }
         }
       }
@@ -1048,36 +1169,42 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
                         unsigned long long* stats, const ScreenArgs* sc,
                         hipStream_t s) {
   if (n_base == 0) return hipSuccess;
-  const ScreenArgs scv = sc ? *sc : ScreenArgs{nullptr, nullptr, 0, 0.0f};
-#define PG_PRUNE(DP, NC, NK)                                                 \
-  prune_kernel<DP, NC, NK><<<n_base, 64, 0, s>>>(base, n_base, dim, rows, D, \
-                                                 gt, winner, gt_k, mp,       \
-                                                 pruned, pcnt, stats, scv)
-#define PG_PRUNE_D(NC, NK)                    \
-  do {                                        \
-    if (D <= 128) PG_PRUNE(128, NC, NK);      \
-    else if (D <= 256) PG_PRUNE(256, NC, NK); \
-    else PG_PRUNE(512, NC, NK);               \
+  const ScreenArgs scv =
+      sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, 0, 0.0f};
+#define PG_PRUNE(DP, NC, NK, SP)                                           \
+  prune_kernel<DP, NC, NK, SP><<<n_base, 64, 0, s>>>(                      \
+      base, n_base, dim, rows, D, gt, winner, gt_k, mp, pruned, pcnt, stats, \
+      scv)
+#define PG_PRUNE_D(NC, NK, SP)                        \
+  do {                                                \
+    if (D <= 256) PG_PRUNE(256, NC, NK, SP);          \
+    else PG_PRUNE(512, NC, NK, SP);                   \
   } while (0)
-#define PG_PRUNE_K(NC)                        \
-  do {                                        \
-    if (mp <= 32) PG_PRUNE_D(NC, 2);          \
-    else if (mp <= 64) PG_PRUNE_D(NC, 4);     \
-    else PG_PRUNE_D(NC, 8);                   \
+#define PG_PRUNE_K(NC, SP)                            \
+  do {                                                \
+    if (mp <= 32) PG_PRUNE_D(NC, 2, SP);              \
+    else if (mp <= 64) PG_PRUNE_D(NC, 4, SP);         \
+    else PG_PRUNE_D(NC, 8, SP);                       \
+  } while (0)
+#define PG_PRUNE_S(NC)                                \
+  do {                                                \
+    if (scv.base16lo) PG_PRUNE_K(NC, true);           \
+    else PG_PRUNE_K(NC, false);                       \
   } while (0)
   if (D > 512) return hipErrorInvalidValue;
   if (sc == nullptr) {
-    if (D <= 64) PG_PRUNE(64, 0, 1);
-    else if (D <= 128) PG_PRUNE(128, 0, 1);
-    else if (D <= 256) PG_PRUNE(256, 0, 1);
-    else PG_PRUNE(512, 0, 1);
+    if (D <= 64) PG_PRUNE(64, 0, 1, false);
+    else if (D <= 128) PG_PRUNE(128, 0, 1, false);
+    else if (D <= 256) PG_PRUNE(256, 0, 1, false);
+    else PG_PRUNE(512, 0, 1, false);
   } else if (sc->dpad <= 256) {
-    PG_PRUNE_K(8);
+    PG_PRUNE_S(8);
   } else if (sc->dpad <= 512) {
-    PG_PRUNE_K(16);
+    PG_PRUNE_S(16);
   } else {
     return hipErrorInvalidValue;
   }
+#undef PG_PRUNE_S
 #undef PG_PRUNE_K
 #undef PG_PRUNE_D
 #undef PG_PRUNE
@@ -1112,26 +1239,33 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
                         int strict, uint32_t* out, unsigned long long* stats,
                         const ScreenArgs* sc, hipStream_t s) {
   if (n_base == 0) return hipSuccess;
-  const ScreenArgs scv = sc ? *sc : ScreenArgs{nullptr, nullptr, 0, 0.0f};
-#define PG_MERGE(NC, NK)                                                   \
-  merge_kernel<NC, NK><<<n_base, 64, 0, s>>>(base, n_base, dim, off, srcs, \
-                                             cap, pruned, pcnt, mp, deg,   \
-                                             strict, out, stats, scv)
-#define PG_MERGE_K(NC)                    \
-  do {                                    \
-    if (deg <= 32) PG_MERGE(NC, 2);       \
-    else if (deg <= 64) PG_MERGE(NC, 4);  \
-    else PG_MERGE(NC, 8);                 \
+  const ScreenArgs scv =
+      sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, 0, 0.0f};
+#define PG_MERGE(NC, NK, SP)                                                 \
+  merge_kernel<NC, NK, SP><<<n_base, 64, 0, s>>>(base, n_base, dim, off,     \
+                                                 srcs, cap, pruned, pcnt, mp, \
+                                                 deg, strict, out, stats, scv)
+#define PG_MERGE_K(NC, SP)                    \
+  do {                                        \
+    if (deg <= 32) PG_MERGE(NC, 2, SP);       \
+    else if (deg <= 64) PG_MERGE(NC, 4, SP);  \
+    else PG_MERGE(NC, 8, SP);                 \
+  } while (0)
+#define PG_MERGE_S(NC)                        \
+  do {                                        \
+    if (scv.base16lo) PG_MERGE_K(NC, true);   \
+    else PG_MERGE_K(NC, false);               \
   } while (0)
   if (sc == nullptr || !strict) {
-    PG_MERGE(0, 1);  // the non-strict fill runs on the plain kernel only
+    PG_MERGE(0, 1, false);  // the non-strict fill runs on the plain kernel
   } else if (sc->dpad <= 256) {
-    PG_MERGE_K(8);
+    PG_MERGE_S(8);
   } else if (sc->dpad <= 512) {
-    PG_MERGE_K(16);
+    PG_MERGE_S(16);
   } else {
     return hipErrorInvalidValue;
   }
+#undef PG_MERGE_S
 #undef PG_MERGE_K
 #undef PG_MERGE
   return hipGetLastError();
@@ -1220,8 +1354,30 @@ static hipError_t launch_bf16(const void* base16, uint32_t dim, uint32_t dpad,
                             table_size, out, stats, queue, s);
 }
 
+template <int NCH, int G>
+static hipError_t launch_fp8(const void* base8, const float* scale8,
+                             uint32_t dim, uint32_t dpad, const float* queries,
+                             uint32_t n_query, uint32_t n_base,
+                             const uint32_t* graph, uint32_t gdeg, uint32_t Km,
+                             uint32_t Kp, uint32_t Kd, uint32_t topk,
+                             uint32_t reset_iter, uint32_t entry_seed,
+                             uint32_t max_iters, uint32_t table_size,
+                             uint32_t* out, unsigned long long* stats,
+                             uint32_t* queue, hipStream_t s) {
+  EvalFP8<NCH, G> ev;
+  ev.base8 = reinterpret_cast<const uint8_t*>(base8);
+  ev.scale8 = scale8;
+  ev.dim = dim;
+  ev.dpad = dpad;
+  ev.nch = dpad / 32;
+  return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
+                            Kd, topk, reset_iter, entry_seed, max_iters,
+                            table_size, out, stats, queue, s);
+}
+
 hipError_t launch_search(const float* queries, uint32_t n_query,
-                         const float* base, const void* base16, uint32_t dpad,
+                         const float* base, const void* base16,
+                         const void* base8, const float* scale8, uint32_t dpad,
                          uint32_t n_base, uint32_t dim,
                          const uint32_t* graph, uint32_t gdeg, uint32_t Km,
                          uint32_t Kp, uint32_t Kd, uint32_t topk,
@@ -1234,8 +1390,18 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
 #define PG_ARGS                                                             \
   queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,      \
       entry_seed, max_iters, table_size, out, stats, queue, s
+  if (base8 != nullptr) {
+    const uint32_t nch = dpad / 32;
+    const bool two = gather_batch >= 2;
+    if (nch <= 8)
+      return two ? launch_fp8<8, 2>(base8, scale8, dim, dpad, PG_ARGS)
+                 : launch_fp8<8, 1>(base8, scale8, dim, dpad, PG_ARGS);
+    if (nch <= 16)
+      return two ? launch_fp8<16, 2>(base8, scale8, dim, dpad, PG_ARGS)
+                 : launch_fp8<16, 1>(base8, scale8, dim, dpad, PG_ARGS);
+    return hipErrorInvalidValue;
+  }
   if (base16 != nullptr) {
-    // screened path: gather_batch = groups of 16 candidates in flight
     // NCH chunks of 32 (8 or 16; chunks past dpad/32 are zero and skip
     // their loads); gather_batch >= 2 puts two 16-candidate groups in flight
     const uint32_t nch = dpad / 32;
@@ -1270,10 +1436,12 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
 }
 
 // Screened-path preparation, one wave per row: the BF16 copy of the row
-// (round to nearest even, zero-padded to dpad) and its FP32 norm.
+// (round to nearest even, zero-padded to dpad), optionally the residual
+// copy lo = bf16(x - hi) for the split screen, and the FP32 norm.
 __global__ __launch_bounds__(256) void prepare_screen_kernel(
     const float* __restrict__ base, uint32_t n, uint32_t dim, uint32_t dpad,
-    __bf16* __restrict__ out16, float* __restrict__ norms) {
+    __bf16* __restrict__ out16, __bf16* __restrict__ lo16,
+    float* __restrict__ norms) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wpb = blockDim.x / 64u;
   for (uint32_t r = blockIdx.x * wpb + threadIdx.x / 64u; r < n;
@@ -1282,7 +1450,9 @@ __global__ __launch_bounds__(256) void prepare_screen_kernel(
     float ss = 0.0f;
     for (uint32_t c = lane; c < dpad; c += 64u) {
       const float v = c < dim ? x[c] : 0.0f;
-      out16[size_t(r) * dpad + c] = (__bf16)v;
+      const __bf16 h = (__bf16)v;
+      out16[size_t(r) * dpad + c] = h;
+      if (lo16) lo16[size_t(r) * dpad + c] = (__bf16)(v - (float)h);
       ss = __builtin_fmaf(v, v, ss);
     }
     ss = wave_sum(ss);
@@ -1290,13 +1460,52 @@ __global__ __launch_bounds__(256) void prepare_screen_kernel(
   }
 }
 
+// FP8 rows for the FP8-screened search, one wave per row: s = max|x|/448
+// (1 when the row is zero), bytes e4m3(x / s) zero-padded to dpad.
+__global__ __launch_bounds__(256) void prepare_fp8_kernel(
+    const float* __restrict__ base, uint32_t n, uint32_t dim, uint32_t dpad,
+    uint8_t* __restrict__ out8, float* __restrict__ scale) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wpb = blockDim.x / 64u;
+  for (uint32_t r = blockIdx.x * wpb + threadIdx.x / 64u; r < n;
+       r += gridDim.x * wpb) {
+    const float* x = base + size_t(r) * dim;
+    float mx = 0.0f;
+    for (uint32_t c = lane; c < dim; c += 64u) mx = fmaxf(mx, fabsf(x[c]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    const float sc = mx > 0.0f ? mx / 448.0f : 1.0f;
+    const float inv = 1.0f / sc;
+    // 8 consecutive bytes per lane per step
+    for (uint32_t c8 = lane * 8u; c8 < dpad; c8 += 512u) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        v[j] = (c8 + j) < dim ? x[c8 + j] * inv : 0.0f;
+      *reinterpret_cast<long*>(out8 + size_t(r) * dpad + c8) = pack_fp8x8(v);
+    }
+    if (lane == 0) scale[r] = sc;
+  }
+}
+
+hipError_t launch_prepare_fp8(const float* base, uint32_t n, uint32_t dim,
+                              uint32_t dpad, void* out8, float* scale,
+                              hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t blocks = (n + 3) / 4 < 65536u ? (n + 3) / 4 : 65536u;
+  prepare_fp8_kernel<<<blocks, 256, 0, s>>>(
+      base, n, dim, dpad, reinterpret_cast<uint8_t*>(out8), scale);
+  return hipGetLastError();
+}
+
 hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
-                                 uint32_t dpad, void* out16, float* norms,
-                                 hipStream_t s) {
+                                 uint32_t dpad, void* out16, void* lo16,
+                                 float* norms, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t blocks = (n + 3) / 4 < 65536u ? (n + 3) / 4 : 65536u;
   prepare_screen_kernel<<<blocks, 256, 0, s>>>(
-      base, n, dim, dpad, reinterpret_cast<__bf16*>(out16), norms);
+      base, n, dim, dpad, reinterpret_cast<__bf16*>(out16),
+      reinterpret_cast<__bf16*>(lo16), norms);
   return hipGetLastError();
 }
 

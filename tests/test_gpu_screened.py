@@ -142,9 +142,10 @@ def _stage_inputs(n, dim, seed=0, integer=False):
     return datagen.base_vectors(n, dim, 64)
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("integer", [False, True])
 @pytest.mark.parametrize("dim,mp", [(200, 32), (512, 64)])
-def test_screened_prune_and_merge_equal_oracle(integer, dim, mp, cuda):
+def test_screened_prune_and_merge_equal_oracle(split, integer, dim, mp, cuda):
     """BF16 screen + exact fallback must reproduce the FP32 decisions exactly,
     including on data full of exact ties (fallbacks exercised)."""
     import torch
@@ -153,7 +154,8 @@ def test_screened_prune_and_merge_equal_oracle(integer, dim, mp, cuda):
     rng = np.random.default_rng(dim + int(integer))
     rows = rng.integers(0, n, (n, 256)).astype(np.uint32)
     rows[rng.random(rows.shape) < 0.05] = 0x7FFFFFFF
-    p = _native.make_params(n, dim, n, mp, 128, precision=_native.PG_BF16)
+    p = _native.make_params(n, dim, n, mp, 128, precision=_native.PG_BF16,
+                            tune_flags=0x20 | (0x80 if split else 0))
     base_d = _dev(base, torch)
     pr_d = torch.full((n, mp), -1, dtype=torch.int32, device=cuda)
     pc_d = torch.zeros(n, dtype=torch.int32, device=cuda)
@@ -203,3 +205,73 @@ def test_screened_whole_stage_matches_oracle_given_its_search(cuda):
     off, srcs = cpu_ref.reverse_csr(pr, pc)
     link = cpu_ref.merge(w["base"], off, srcs, 126 - M, pr, pc, M, True)
     np.testing.assert_array_equal(_host(link_d), link)
+
+
+@pytest.mark.parametrize("dim", [200, 512])
+def test_fp8_search_complete_graph_recall(dim, cuda):
+    """FP8 e4m3 screening (per-vector scales): on a complete graph the
+    returned top-16 must mostly be the exact top-16, and always among the
+    exact top-32."""
+    import torch
+    rng = np.random.default_rng(100 + dim)
+    n = 64
+    base = rng.standard_normal((n, dim)).astype(np.float32)
+    qs = rng.standard_normal((32, dim)).astype(np.float32)
+    graph = np.full((n, n - 1), TOMB, np.uint32)
+    for v in range(n):
+        graph[v] = [u for u in range(n) if u != v]
+    p = _native.make_params(n, dim, n, 2, 16, Kd=n - 1, Kp=1, topk=16,
+                            proj_deg=n - 1, rounds=1,
+                            precision=_native.PG_FP8E4M3)
+    out_d = torch.empty((qs.shape[0], p.topk), dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.run_search(_dev(qs, torch), _dev(base, torch), _dev(graph, torch),
+                       out_d)
+        ctx.sync()
+    out = _host(out_d)
+    d64 = -(qs.astype(np.float64) @ base.astype(np.float64).T)
+    hits = []
+    for q in range(qs.shape[0]):
+        order = np.argsort(d64[q], kind="stable")
+        got = set(out[q, :16].tolist())
+        assert len(got) == 16
+        assert got <= set(order[:32].tolist())
+        hits.append(len(got & set(order[:16].tolist())) / 16)
+    assert np.mean(hits) >= 0.8, np.mean(hits)
+
+
+def test_fp8_build_and_prune(cuda):
+    import torch
+    n, dim, M, L = 3000, 512, 64, 160
+    w = datagen.make_workload(n, dim, 256, normalize=True, device="cpu")
+    gt = np.ascontiguousarray(w["gt"].view(np.uint32))
+    base_d, gt_d = _dev(w["base"], torch), _dev(gt, torch)
+    links = {}
+    for prec in (_native.PG_FP32, _native.PG_FP8E4M3):
+        p = _native.make_params(n, dim, n, M, L, precision=prec)
+        proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=cuda)
+        link_d = torch.empty((n, M), dtype=torch.int32, device=cuda)
+        with _native.Context(p) as ctx:
+            ctx.stage_device(base_d, gt_d, proj_d, link_d)
+            ctx.wait()
+            assert ctx.stats().search_capped == 0
+        links[prec] = _host(link_d)
+    a, b = links[_native.PG_FP32], links[_native.PG_FP8E4M3]
+    valid = b != TOMB
+    assert (b[valid] < n).all()
+    overlap = np.mean([len(set(x[x != TOMB]) & set(y[y != TOMB])) /
+                       max(1, len(set(x[x != TOMB]))) for x, y in zip(a, b)])
+    assert overlap > 0.6, overlap
+    # the FP8 build's prune is the BF16-screened exact prune
+    rng = np.random.default_rng(2)
+    rows = rng.integers(0, n, (n, 320)).astype(np.uint32)
+    p = _native.make_params(n, dim, n, M, L, precision=_native.PG_FP8E4M3)
+    pr_d = torch.full((n, M), -1, dtype=torch.int32, device=cuda)
+    pc_d = torch.zeros(n, dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.run_prune_rows(base_d, _dev(rows, torch), M, pr_d, pc_d)
+        ctx.sync()
+    pr, pc = cpu_ref.prune(w["base"], rows, M)
+    np.testing.assert_array_equal(_host(pc_d), pc)
+    mask = np.arange(M)[None, :] < pc[:, None]
+    np.testing.assert_array_equal(_host(pr_d)[mask], pr[mask])
