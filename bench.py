@@ -145,6 +145,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
+    ap.add_argument("--no-dist-error", action="store_true")
     ap.add_argument("--n-eval", type=int, default=10000)
     ap.add_argument("--tune-flags", type=int, default=None,
                     help="perf-only knobs for the FP32 build (default: the "
@@ -155,6 +156,10 @@ def main():
                     help="overrides of the BF16 build's knobs, e.g. "
                          "'Kp=16,table_size=2048' (beam knobs change recall; "
                          "the FP32 build always uses the reference mapping)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the perf-only knobs (gather batch, persistent "
+                         "grid, visited-cache size, split screen) per "
+                         "precision by timing them on this workload")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
@@ -222,6 +227,16 @@ def main():
             p = _native.make_params(
                 n, dim, n, M, L, precision=_native.PG_BF16 if prec == "bf16"
                 else _native.PG_FP8E4M3, **kw)
+        tuned = None
+        if args.autotune:
+            from paragraph_amd import autotune
+            best, rep = autotune.autotune(base_d, gt_d, p, use_cache=False,
+                                          log=log)
+            p = autotune.apply(p, best)
+            if prec == "fp32":
+                p32 = p
+            tuned = {"knobs": best, "seconds": rep["seconds"]}
+            log(f"{prec}: autotuned knobs {best} in {rep['seconds']} s")
         proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
         link_d = torch.empty((n, M), dtype=torch.int32, device=dev)
         ctx = _native.Context(p)
@@ -251,6 +266,10 @@ def main():
                 eval_ctx, eq_d, base_d, link_d, truth, p32.topk), 4)
             log(f"{prec}: recall@10 of the link graph "
                 f"({truth.shape[0]} held-out queries) = {r['recall_at_10']}")
+        if tuned is not None:
+            r["autotune"] = tuned
+        r["tune_flags"] = int(p.tune_flags)
+        r["table_size"] = int(p.table_size)
         r["link"] = link_d
         r["ctx"] = ctx
         results[prec] = r
@@ -264,12 +283,25 @@ def main():
                            args.cpu_sample, log)
 
     screened = [x for x in precs if x != "fp32"]
+    dist_err = None
+    if not args.no_dist_error:
+        ev = {}
+        ctx32 = results["fp32"]["ctx"] if "fp32" in results else \
+            _native.Context(p32)
+        ev["fp32_search"] = (ctx32, False)
+        ev["fp32_chain"] = (ctx32, True)
+        for sp in screened:
+            ev[sp + "_screen"] = (results[sp]["ctx"], False)
+        dist_err = distance_error(ev, w, base_d)
+        log(f"distance error vs FP64: {dist_err}")
     head_prec = screened[0] if screened else "fp32"
     head = results[head_prec]
     extra = {}
     for prec, r in results.items():
         extra[prec] = {k: (round(v, 3) if isinstance(v, float) else v)
                        for k, v in r.items() if k not in ("link", "ctx")}
+    if dist_err is not None:
+        extra["distance_error_vs_fp64"] = dist_err
     for sp in screened:
         if "fp32" not in results:
             break
@@ -317,6 +349,43 @@ def main():
     print(json.dumps(line), flush=True)
     for r in results.values():
         r["ctx"].close()
+
+
+def distance_error(ctxs, w, base_d, n_sample=2000):
+    """Error of each distance evaluator against FP64 on (training query,
+    exact top-128 id) pairs: |d - d64| / (|x| |y|) (mean, p99, max), and the
+    top-10 overlap inside each 128-id row between the evaluator's order and
+    the FP64 order."""
+    import torch
+    nq = min(n_sample, w["queries"].shape[0])
+    q_d = torch.from_numpy(np.ascontiguousarray(w["queries"][:nq])).to(
+        base_d.device)
+    ids = np.ascontiguousarray(w["gt"][:nq])
+    ids_d = torch.from_numpy(ids).to(base_d.device)
+    valid = ids_d >= 0
+    safe = torch.where(valid, ids_d, torch.zeros_like(ids_d)).long()
+    rows = base_d[safe].double()                    # nq x k x dim
+    q64 = q_d.double()
+    d64 = -(rows @ q64[:, :, None]).squeeze(2)
+    norm = rows.norm(dim=2) * q64.norm(dim=1, keepdim=True)
+    big = torch.tensor(float("inf"), dtype=torch.float64, device=d64.device)
+    top64 = torch.topk(torch.where(valid, -d64, -big), 10, dim=1).indices
+    out = {}
+    for name, (ctx, chain) in ctxs.items():
+        got = torch.empty(ids.shape, dtype=torch.float32, device=base_d.device)
+        ctx.run_pair_distances(q_d, base_d, ids_d, got, chain=chain)
+        ctx.sync()
+        g = got.double()
+        rel = ((g - d64).abs() / norm.clamp_min(1e-300))[valid]
+        topg = torch.topk(torch.where(valid, -g, -big), 10, dim=1).indices
+        ov = (topg[:, :, None] == top64[:, None, :]).any(2).double().mean()
+        out[name] = {"rel_err_mean": float(rel.mean()),
+                     "rel_err_p99": float(torch.quantile(
+                         rel[:min(rel.numel(), 1 << 24)].float(), 0.99)),
+                     "rel_err_max": float(rel.max()),
+                     "top10_overlap_in_gt128": round(float(ov), 4)}
+    out["pairs"] = int(valid.sum())
+    return out
 
 
 def cpu_baseline(ctx, p, w, gt_u32, base_d, gt_d, sample, log):

@@ -72,7 +72,9 @@ typedef struct pg_params {
   uint32_t entry_seed;  /* entry points (seed + i) mod N (3030517) */
   uint32_t max_iters;   /* search iteration watchdog (not in the reference) */
   uint32_t precision;   /* pg_precision */
-  uint32_t rerank;      /* screened paths: FP32 re-rank width (0 = topk) */
+  uint32_t rerank;      /* screened paths: the search keeps its first rerank
+                           entries (screened order) and the prune re-ranks
+                           them in FP32 (0 = topk) */
   uint32_t tune_flags;  /* perf-only knobs, 0 = defaults.  bits 0-4: search
                            gather batch (FP32: candidate pairs in flight;
                            BF16: 16-candidate groups); bit 5 (0x20):
@@ -184,6 +186,18 @@ int pg_run_merge(pg_ctx* ctx, const float* d_base, const uint32_t* d_off,
 int pg_run_search(pg_ctx* ctx, const float* d_queries, uint32_t n_query,
                   const float* d_base, const uint32_t* d_graph,
                   uint32_t graph_deg, uint32_t* d_out);
+
+/* Distances of (query, base id) pairs, n_query x k ids (k <= 1024), written
+ * as n_query x k floats (NaN for ids >= n_base).  kind 0: the search
+ * stage's evaluator at the context's precision (FP32: the per-lane chains
+ * and 16/8/4/2/1 tree of Gsearcher.cuh:1022-1031; BF16 / FP8: the MFMA
+ * screening distance); kind 1: the FP32 chain of the prune and merge stages
+ * (Gbuilder.cuh:785-797).  Measurement hook for the screening error; the
+ * reference has no separate entry for it. */
+int pg_run_pair_distances(pg_ctx* ctx, const float* d_queries,
+                          uint32_t n_query, const float* d_base,
+                          const uint32_t* d_ids, uint32_t k, int kind,
+                          float* d_out);
 
 int pg_sync(pg_ctx* ctx);  /* hipStreamSynchronize + error check */
 

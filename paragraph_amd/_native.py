@@ -49,7 +49,7 @@ EXPORTED = (
     "pg_gpu_stage_async", "pg_gpu_stage_device_async", "pg_wait", "pg_query",
     "pg_get_stats", "pg_reset_stats", "pg_get_stage_counters", "pg_last_error", "pg_stage_name",
     "pg_run_match", "pg_run_prune", "pg_run_reverse", "pg_run_merge",
-    "pg_run_search", "pg_sync",
+    "pg_run_search", "pg_run_pair_distances", "pg_sync",
 )
 
 _lib = None
@@ -99,6 +99,8 @@ def _declare(L):
     L.pg_run_merge.argtypes = [_P, _P, _P, _P, _U32, _P, _P, _U32, _U32,
                                ctypes.c_int, _P]
     L.pg_run_search.argtypes = [_P, _P, _U32, _P, _P, _U32, _P]
+    L.pg_run_pair_distances.argtypes = [_P, _P, _U32, _P, _P, _U32,
+                                        ctypes.c_int, _P]
 
 
 def stage_name(i):
@@ -228,6 +230,15 @@ class Context:
         _check(lib().pg_run_search(self._ctx, _dptr(queries), queries.shape[0],
                                    _dptr(base), _dptr(graph), graph.shape[1],
                                    _dptr(out)), self._ctx)
+
+    def run_pair_distances(self, queries, base, ids, out, chain=False):
+        """out[q, t] = distance of queries[q] to base[ids[q, t]]; chain=False:
+        the search evaluator at this context's precision, chain=True: the
+        FP32 prune chain."""
+        _check(lib().pg_run_pair_distances(
+            self._ctx, _dptr(queries), queries.shape[0], _dptr(base),
+            _dptr(ids), ids.shape[1], 1 if chain else 0, _dptr(out)),
+            self._ctx)
 
     def sync(self):
         _check(lib().pg_sync(self._ctx), self._ctx)

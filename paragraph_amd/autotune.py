@@ -1,0 +1,141 @@
+"""Autotuner for the perf-only knobs of the index build.
+
+Knobs (pg_params.tune_flags and table_size):
+  - gather batch: FP32 search, candidate pairs in flight (4, 8 or 16);
+    screened search, 16-candidate MFMA groups in flight (1 or 2);
+  - persistent search grid on / off;
+  - split-BF16 screen in the screened prune / merge on / off;
+  - visited-table size (FP32: exact set, any size above the bound gives the
+    same rows; screened: visited-cache size, 2x / 4x / 8x (Km + Kp*Kd)).
+Beam knobs (Km, Kp, Kd, rerank) change results and are not tuned here.
+
+Method: one default build gives a realistic graph; then the search stage is
+timed on a sample of base rows for every search knob set, and the prune
+stage on the resulting rows for the screen variants.  The fastest set wins.
+Results are cached in a JSON file keyed by shape, precision and device.
+"""
+import json
+import os
+import time
+
+from . import _native
+
+_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "paragraph_amd",
+                      "tune.json")
+
+
+def _key(p, device_name):
+    return (f"{device_name}|n={p.n_base}|dim={p.dim}|M={p.M}|L={p.Km}|"
+            f"Kp={p.Kp}|prec={p.precision}")
+
+
+def _time(fn, reps=2):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def candidates(p):
+    """Knob sets to try for params p (tune_flags, table_size)."""
+    screened = p.precision != _native.PG_FP32
+    kpkd = p.Km + p.Kp * p.Kd
+    out = []
+    gathers = (1, 2) if screened else (4, 8, 16)
+    tables = ([0, 2 * kpkd, 8 * kpkd] if screened else [0])
+    for g in gathers:
+        for persistent in (True, False):
+            for ts in tables:
+                tf = g | (0x20 if persistent else 0)
+                out.append({"tune_flags": tf, "table_size": ts})
+    return out
+
+
+def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
+    """Returns ({"tune_flags", "table_size"}, report).  base_d / gt_d are
+    device tensors of the workload; p is the pg_params to tune."""
+    import torch
+    dev_name = torch.cuda.get_device_name(0)
+    key = _key(p, dev_name)
+    cache = {}
+    if use_cache and os.path.exists(_CACHE):
+        try:
+            cache = json.load(open(_CACHE))
+        except (OSError, ValueError):
+            cache = {}
+    if use_cache and key in cache:
+        return cache[key]["best"], cache[key]
+    t_start = time.perf_counter()
+    n = p.n_base
+    i32 = dict(dtype=torch.int32, device=base_d.device)
+    proj = torch.empty((n, p.proj_deg), **i32)
+    link = torch.empty((n, p.M), **i32)
+    with _native.Context(p) as ctx:
+        ctx.stage_device(base_d, gt_d, proj, link)
+        ctx.wait()
+    m = min(sample, n)
+    q = base_d[:m].contiguous()
+    report = {"search": [], "prune": []}
+    best, best_t = None, float("inf")
+    for kn in candidates(p):
+        pp = _native.PgParams.from_buffer_copy(p)
+        pp.tune_flags = (p.tune_flags & ~0x3F) | kn["tune_flags"]
+        pp.table_size = kn["table_size"]
+        if _native.check_params(pp)[0] != 0:
+            continue
+        out = torch.empty((m, pp.topk), **i32)
+        with _native.Context(pp) as ctx:
+            t = _time(lambda: (ctx.run_search(q, base_d, link, out),
+                               ctx.sync()))
+        report["search"].append({**kn, "ms": round(1000 * t, 3)})
+        if log:
+            log(f"autotune search {kn}: {1000 * t:.2f} ms")
+        if t < best_t:
+            best, best_t = dict(kn), t
+    if p.precision != _native.PG_FP32:
+        rows = torch.empty((m, p.topk), **i32)
+        with _native.Context(p) as ctx:
+            ctx.run_search(q, base_d, link, rows)
+            ctx.sync()
+        full = torch.full((n, p.topk), -1, **i32)
+        full[:m] = rows
+        pr = torch.empty((n, p.M), **i32)
+        pc = torch.empty(n, **i32)
+        best_split, best_pt = 0, float("inf")
+        for split in (0, 0x80):
+            pp = _native.PgParams.from_buffer_copy(p)
+            pp.tune_flags = (p.tune_flags & ~0x80) | split
+            with _native.Context(pp) as ctx:
+                t = _time(lambda: (ctx.run_prune_rows(base_d, full, p.M, pr,
+                                                      pc), ctx.sync()))
+            report["prune"].append({"split": bool(split),
+                                    "ms": round(1000 * t, 3)})
+            if log:
+                log(f"autotune prune split={bool(split)}: {1000 * t:.2f} ms")
+            if t < best_pt:
+                best_split, best_pt = split, t
+        best["tune_flags"] |= best_split
+    report["best"] = best
+    report["seconds"] = round(time.perf_counter() - t_start, 2)
+    if use_cache:
+        cache[key] = report
+        try:
+            os.makedirs(os.path.dirname(_CACHE), exist_ok=True)
+            json.dump(cache, open(_CACHE, "w"), indent=1)
+        except OSError:
+            pass
+    return best, report
+
+
+def apply(p, best):
+    """Copy of p with the tuned knobs."""
+    pp = _native.PgParams.from_buffer_copy(p)
+    pp.tune_flags = (p.tune_flags & ~0xBF) | best["tune_flags"]
+    pp.table_size = best["table_size"]
+    return pp

@@ -239,12 +239,16 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   PG_TRY(record(c, ev++));
   const uint32_t* graph = d_proj;
   uint32_t gdeg = p.proj_deg;
+  // Screened paths: the search emits its first `rerank` entries (ordered by
+  // screened distance) and the prune re-ranks exactly those in FP32.
+  const uint32_t width =
+      (c->base16 && p.rerank && p.rerank < p.topk) ? p.rerank : p.topk;
   for (uint32_t r = 0; r < p.rounds; ++r) {
     PG_HIP(c, pg::launch_search(d_base, N, d_base,
                                 c->base8 ? nullptr : c->base16, c->base8,
                                 c->scale8, c->dpad, N,
                                 p.dim, graph, gdeg, p.Km,
-                                p.Kp, p.Kd, p.topk, p.reset_iter,
+                                p.Kp, p.Kd, width, p.reset_iter,
                                 p.entry_seed, p.max_iters,
                                 effective_table_size(p), tune_gather(p),
                                 c->res, c->stats,
@@ -252,7 +256,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
                                                                  : nullptr,
                                 s));
     PG_TRY(record(c, ev++));
-    PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, p.topk, nullptr,
+    PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, width, nullptr,
                                nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
                                scp, s));
     PG_TRY(record(c, ev++));
@@ -389,6 +393,7 @@ int pg_params_check(const pg_params* p, char* msg, uint32_t msg_len) {
   }
   if (p->rounds > 3) return fail("rounds must be <= 3");
   if (p->max_iters == 0) return fail("max_iters must be > 0");
+  if (p->rerank > p->topk) return fail("rerank must be <= topk (0 = topk)");
   if (p->precision != PG_FP32 && p->precision != PG_BF16 &&
       p->precision != PG_FP8E4M3)
     return fail("precision must be PG_FP32, PG_BF16 or PG_FP8E4M3");
@@ -709,6 +714,25 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
                               (p.tune_flags & kTunePersistent) ? c->queue
                                                                : nullptr,
                               c->stream));
+  return PG_OK;
+}
+
+int pg_run_pair_distances(pg_ctx* c, const float* d_queries,
+                          uint32_t n_query, const float* d_base,
+                          const uint32_t* d_ids, uint32_t k, int kind,
+                          float* d_out) {
+  if (!c || !d_base || (n_query && (!d_queries || !d_ids || !d_out)))
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  if (k > 1024) return set_err(c, PG_ERR_INVALID, "k (%u) exceeds 1024", k);
+  if (kind != 0 && kind != 1)
+    return set_err(c, PG_ERR_INVALID, "kind must be 0 (search) or 1 (chain)");
+  const pg_params& p = c->p;
+  PG_HIP(c, hipSetDevice(c->device));
+  if (kind == 0 && c->base16) PG_TRY(prepare_screen(c, d_base));
+  PG_HIP(c, pg::launch_pair_dist(d_queries, n_query, d_base,
+                                 c->base8 ? nullptr : c->base16, c->base8,
+                                 c->scale8, c->dpad, p.n_base, p.dim, d_ids,
+                                 k, kind, d_out, c->stream));
   return PG_OK;
 }
 

@@ -63,4 +63,13 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          unsigned long long* stats, uint32_t* queue,
                          hipStream_t s);
 
+// Pair distances: chain != 0 gives the FP32 prune chain; else the search
+// evaluator selected like launch_search (k <= 1024).
+hipError_t launch_pair_dist(const float* queries, uint32_t n_query,
+                            const float* base, const void* base16,
+                            const void* base8, const float* scale8,
+                            uint32_t dpad, uint32_t n_base, uint32_t dim,
+                            const uint32_t* ids, uint32_t k, int chain,
+                            float* out, hipStream_t s);
+
 }  // namespace pg

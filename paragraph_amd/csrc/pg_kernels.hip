@@ -1510,3 +1510,158 @@ hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
 }
 
 }  // namespace pg
+
+namespace pg {
+
+// Distances of given (query, base id) pairs through the search's own
+// evaluators: one one-wave workgroup per query, its k ids staged in LDS,
+// distances written as floats (NaN for ids >= n_base).  Used to measure the
+// screening error of the BF16 / FP8 paths and the FP32 summation orders
+// against FP64.
+constexpr uint32_t kPairMaxK = 1024;
+
+template <typename Eval>
+__global__ __launch_bounds__(64) void pair_dist_kernel(
+    Eval ev, const float* __restrict__ queries, uint32_t n_query,
+    const uint32_t* __restrict__ ids, uint32_t k, uint32_t n_base,
+    float* __restrict__ out) {
+  __shared__ uint32_t cand[kPairMaxK];
+  __shared__ uint32_t todo[kPairMaxK];
+  __shared__ uint64_t tail[kPairMaxK];
+  __shared__ uint32_t ntodo_s;
+  const uint32_t q = blockIdx.x;
+  const uint32_t lane = lane_id();
+  if (lane == 0) ntodo_s = 0;
+  __syncthreads();
+  for (uint32_t t0 = 0; t0 < k; t0 += kWave) {
+    const uint32_t t = t0 + lane;
+    const uint32_t id = t < k ? ids[size_t(q) * k + t] : kTomb;
+    const bool ok = t < k && id < n_base;
+    if (t < k) {
+      cand[t] = ok ? id : 0u;
+      tail[t] = kPadKey;
+    }
+    const uint64_t bal = __ballot(ok);
+    const uint32_t base_pos = ntodo_s;
+    __syncthreads();
+    if (ok) todo[base_pos + __popcll(bal & lanemask_lt())] = t;
+    if (lane == 0) ntodo_s = base_pos + __popcll(bal);
+    __syncthreads();
+  }
+  const uint32_t ntodo = ntodo_s;
+  ev.load_query(queries, q);
+  ev.eval(cand, todo, ntodo, tail);
+  __syncthreads();
+  for (uint32_t t = lane; t < k; t += kWave) {
+    const uint64_t key = tail[t];
+    out[size_t(q) * k + t] = key == kPadKey ? __builtin_nanf("") : key_dist(key);
+  }
+}
+
+// FP32 chain distance of the prune / merge stages for the same pairs.
+__global__ __launch_bounds__(256) void pair_chain_kernel(
+    const float* __restrict__ queries, uint32_t n_query,
+    const float* __restrict__ base, uint32_t dim,
+    const uint32_t* __restrict__ ids, uint32_t k, uint32_t n_base,
+    float* __restrict__ out) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= uint64_t(n_query) * k) return;
+  const uint32_t q = uint32_t(i / k);
+  const uint32_t id = ids[i];
+  out[i] = id < n_base ? chain_dist(queries + size_t(q) * dim,
+                                    base + size_t(id) * dim, dim)
+                       : __builtin_nanf("");
+}
+
+template <typename Eval>
+static hipError_t launch_pairs_eval(const Eval& ev, const float* queries,
+                                    uint32_t n_query, const uint32_t* ids,
+                                    uint32_t k, uint32_t n_base, float* out,
+                                    hipStream_t s) {
+  pair_dist_kernel<Eval><<<n_query, 64, 0, s>>>(ev, queries, n_query, ids, k,
+                                                n_base, out);
+  return hipGetLastError();
+}
+
+template <int TQ>
+static hipError_t pairs_f32(const float* base, uint32_t dim,
+                            const float* queries, uint32_t n_query,
+                            const uint32_t* ids, uint32_t k, uint32_t n_base,
+                            float* out, hipStream_t s) {
+  EvalF32<TQ, 4> ev;
+  ev.base = base;
+  ev.dim = dim;
+  return launch_pairs_eval(ev, queries, n_query, ids, k, n_base, out, s);
+}
+
+template <int NCH>
+static hipError_t pairs_bf16(const void* base16, uint32_t dim, uint32_t dpad,
+                             const float* queries, uint32_t n_query,
+                             const uint32_t* ids, uint32_t k, uint32_t n_base,
+                             float* out, hipStream_t s) {
+  EvalBF16<NCH, 1> ev;
+  ev.base16 = reinterpret_cast<const __bf16*>(base16);
+  ev.dim = dim;
+  ev.dpad = dpad;
+  ev.nch = dpad / 32;
+  return launch_pairs_eval(ev, queries, n_query, ids, k, n_base, out, s);
+}
+
+template <int NCH>
+static hipError_t pairs_fp8(const void* base8, const float* scale8,
+                            uint32_t dim, uint32_t dpad, const float* queries,
+                            uint32_t n_query, const uint32_t* ids, uint32_t k,
+                            uint32_t n_base, float* out, hipStream_t s) {
+  EvalFP8<NCH, 1> ev;
+  ev.base8 = reinterpret_cast<const uint8_t*>(base8);
+  ev.scale8 = scale8;
+  ev.dim = dim;
+  ev.dpad = dpad;
+  ev.nch = dpad / 32;
+  return launch_pairs_eval(ev, queries, n_query, ids, k, n_base, out, s);
+}
+
+hipError_t launch_pair_dist(const float* queries, uint32_t n_query,
+                            const float* base, const void* base16,
+                            const void* base8, const float* scale8,
+                            uint32_t dpad, uint32_t n_base, uint32_t dim,
+                            const uint32_t* ids, uint32_t k, int chain,
+                            float* out, hipStream_t s) {
+  if (n_query == 0 || k == 0) return hipSuccess;
+  if (k > kPairMaxK) return hipErrorInvalidValue;
+  if (chain) {
+    const uint64_t total = uint64_t(n_query) * k;
+    const uint64_t blocks = (total + 255) / 256;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    pair_chain_kernel<<<uint32_t(blocks), 256, 0, s>>>(
+        queries, n_query, base, dim, ids, k, n_base, out);
+    return hipGetLastError();
+  }
+  const uint32_t nch = dpad / 32;
+  if (base8 != nullptr) {
+    if (nch <= 8)
+      return pairs_fp8<8>(base8, scale8, dim, dpad, queries, n_query, ids, k,
+                          n_base, out, s);
+    if (nch <= 16)
+      return pairs_fp8<16>(base8, scale8, dim, dpad, queries, n_query, ids, k,
+                           n_base, out, s);
+    return hipErrorInvalidValue;
+  }
+  if (base16 != nullptr) {
+    if (nch <= 8)
+      return pairs_bf16<8>(base16, dim, dpad, queries, n_query, ids, k, n_base,
+                           out, s);
+    if (nch <= 16)
+      return pairs_bf16<16>(base16, dim, dpad, queries, n_query, ids, k,
+                            n_base, out, s);
+    return hipErrorInvalidValue;
+  }
+  const uint32_t tq = (dim + 31) / 32;
+  if (tq <= 4) return pairs_f32<4>(base, dim, queries, n_query, ids, k, n_base, out, s);
+  if (tq <= 8) return pairs_f32<8>(base, dim, queries, n_query, ids, k, n_base, out, s);
+  if (tq <= 16) return pairs_f32<16>(base, dim, queries, n_query, ids, k, n_base, out, s);
+  if (tq <= 32) return pairs_f32<32>(base, dim, queries, n_query, ids, k, n_base, out, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace pg

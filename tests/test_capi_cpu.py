@@ -81,3 +81,21 @@ def test_stage_names():
     assert _native.stage_name(0) == "match"
     assert _native.stage_name(4) == "search_round1"
     assert _native.stage_name(11) == "merge_round2"
+
+
+def test_autotune_candidates_valid():
+    """Every knob set the autotuner tries passes the parameter check and
+    changes only perf knobs."""
+    from paragraph_amd import autotune
+    for prec in (_native.PG_FP32, _native.PG_BF16, _native.PG_FP8E4M3):
+        p = _native.make_params(100000, 200, 100000, 32, 128, precision=prec)
+        cands = autotune.candidates(p)
+        assert len(cands) >= 4
+        for kn in cands:
+            pp = autotune.apply(p, {**kn, "tune_flags": kn["tune_flags"]})
+            assert _native.check_params(pp)[0] == 0, kn
+            for f in ("M", "Km", "Kp", "Kd", "topk", "rerank", "precision",
+                      "rounds", "reset_iter", "entry_seed"):
+                assert getattr(pp, f) == getattr(p, f)
+        pp = autotune.apply(p, {"tune_flags": 0xA4, "table_size": 0})
+        assert pp.tune_flags == 0xA4
