@@ -86,7 +86,10 @@ def roofline_of(ctx, p, n, workload, precision):
     stage_ms = [float(st.stage_ms[i]) for i in range(st.n_stages)]
     names = [_native.stage_name(i) for i in range(st.n_stages)]
     counters = [ctx.stage_counters(i) for i in range(st.n_stages)]
-    row = 2 * ((p.dim + 31) // 32 * 32) if precision == "bf16" else 4 * p.dim
+    dpad = (p.dim + 31) // 32 * 32
+    # bytes of one base row as the search reads it: padded BF16, padded
+    # e4m3 plus its FP32 scale, or FP32
+    row = {"bf16": 2 * dpad, "fp8": dpad + 4}.get(precision, 4 * p.dim)
     kinds = {}
     for i, nm in enumerate(names):
         kind = nm.split("_")[0]
@@ -156,10 +159,16 @@ def main():
                     help="overrides of the BF16 build's knobs, e.g. "
                          "'Kp=16,table_size=2048' (beam knobs change recall; "
                          "the FP32 build always uses the reference mapping)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="pick the perf-only knobs (gather batch, persistent "
-                         "grid, visited-cache size, split screen) per "
-                         "precision by timing them on this workload")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the autotuner (library defaults; the "
+                         "screened builds then use the FP32 beam mapping)")
+    ap.add_argument("--beam", type=int, default=None,
+                    help="beam width L of the screened builds (Km = L, "
+                         "Kp = ceil(L/16), topk = 2L), skipping the beam "
+                         "tuner")
+    ap.add_argument("--n-tune", type=int, default=10000,
+                    help="tuning queries the beam autotuner scores recall "
+                         "on (a seed disjoint from the eval queries)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
@@ -203,20 +212,48 @@ def main():
     tf16 = {} if args.tune_flags_bf16 is None else {
         "tune_flags": args.tune_flags_bf16}
     p32 = _native.make_params(n, dim, n, M, L, **tf32)
-    eval_ctx = None
-    truth = None
-    if not args.no_recall:
-        eq = w["eval_queries"]
-        eq_d = torch.from_numpy(eq).to(dev)
+
+    def exact_top10(q):
+        q_d = torch.from_numpy(np.ascontiguousarray(q)).to(dev)
         B64 = base_d.double()
         tl = []
-        for s in range(0, eq.shape[0], 1000):
-            sc = eq_d[s:s + 1000].double() @ B64.T
+        for s in range(0, q.shape[0], 1000):
+            sc = q_d[s:s + 1000].double() @ B64.T
             tl.append(torch.topk(sc, 10, dim=1).indices.cpu().numpy())
-        truth = np.concatenate(tl)
-        del B64, sc
+            del sc
+        del B64
         torch.cuda.empty_cache()
-    for prec in precs:
+        return q_d, np.concatenate(tl)
+
+    # the evaluation searcher: reference mapping at the workload's L
+    p_eval = _native.make_params(n, dim, n, M, L)
+    eval_ctx = _native.Context(p_eval)
+    truth = None
+    if not args.no_recall:
+        eq_d, truth = exact_top10(w["eval_queries"])
+    screened_req = [x for x in precs if x != "fp32"]
+    tune_beam = args.autotune and bool(screened_req) and args.beam is None
+    if tune_beam:
+        tq_d, tq_truth = exact_top10(datagen.tune_queries(args.config,
+                                                          args.n_tune))
+
+        def tune_recall(link):
+            return recall_at_10(eval_ctx, tq_d, base_d, link, tq_truth,
+                                p_eval.topk)
+    ref_tune_recall = None
+    if tune_beam and "fp32" not in precs:
+        # the FP32 build's recall on the tuning queries is the reference
+        with _native.Context(p32) as c32:
+            pr_ = torch.empty((n, p32.proj_deg), dtype=torch.int32,
+                              device=dev)
+            lk_ = torch.empty((n, M), dtype=torch.int32, device=dev)
+            c32.stage_device(base_d, gt_d, pr_, lk_)
+            c32.wait()
+            ref_tune_recall = tune_recall(lk_)
+            del pr_, lk_
+    # FP32 first: its recall anchors the beam tuning of the screened builds
+    order = (["fp32"] if "fp32" in precs else []) + screened_req
+    for prec in order:
         if prec == "fp32":
             p = p32
         else:
@@ -228,14 +265,24 @@ def main():
                 n, dim, n, M, L, precision=_native.PG_BF16 if prec == "bf16"
                 else _native.PG_FP8E4M3, **kw)
         tuned = None
+        from paragraph_amd import autotune
+        if prec != "fp32" and args.beam is not None:
+            p = autotune.beam_params(p, args.beam)
         if args.autotune:
-            from paragraph_amd import autotune
+            tuned = {}
+            if prec != "fp32" and tune_beam:
+                p, brep = autotune.tune_beam(base_d, gt_d, p, tune_recall,
+                                             ref_tune_recall, log=log)
+                tuned["beam"] = brep
+                log(f"{prec}: autotuned beam L={brep['L']} in "
+                    f"{brep['seconds']} s")
             best, rep = autotune.autotune(base_d, gt_d, p, use_cache=False,
                                           log=log)
             p = autotune.apply(p, best)
             if prec == "fp32":
                 p32 = p
-            tuned = {"knobs": best, "seconds": rep["seconds"]}
+            tuned["knobs"] = best
+            tuned["knob_seconds"] = rep["seconds"]
             log(f"{prec}: autotuned knobs {best} in {rep['seconds']} s")
         proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=dev)
         link_d = torch.empty((n, M), dtype=torch.int32, device=dev)
@@ -259,17 +306,19 @@ def main():
                           if k != "stage_ms"}}
         log(f"{prec}: {ms:.1f} ms/step, {r['value']:.0f} pts/s; stage ms "
             f"{ {k: round(v, 1) for k, v in detail['stage_ms'].items()} }")
+        if prec == "fp32" and tune_beam:
+            ref_tune_recall = tune_recall(link_d)
         if truth is not None:
-            if eval_ctx is None:
-                eval_ctx = _native.Context(p32)
             r["recall_at_10"] = round(recall_at_10(
-                eval_ctx, eq_d, base_d, link_d, truth, p32.topk), 4)
+                eval_ctx, eq_d, base_d, link_d, truth, p_eval.topk), 4)
             log(f"{prec}: recall@10 of the link graph "
                 f"({truth.shape[0]} held-out queries) = {r['recall_at_10']}")
         if tuned is not None:
             r["autotune"] = tuned
         r["tune_flags"] = int(p.tune_flags)
         r["table_size"] = int(p.table_size)
+        r["beam"] = {"L": int(p.Km), "Kp": int(p.Kp), "Kd": int(p.Kd),
+                     "topk": int(p.topk)}
         r["link"] = link_d
         r["ctx"] = ctx
         results[prec] = r
@@ -339,8 +388,15 @@ def main():
                        "re-rank, BF16-screened exact prune)",
                        "fp8": " (FP8 e4m3 per-vector-scaled MFMA search "
                        "screening, FP32 re-rank, BF16-screened exact "
-                       "prune)"}.get(head_prec, "")),
+                       "prune)"}.get(head_prec, ""))
+                   + (f"; screened beam L={head['beam']['L']}, re-rank "
+                      f"width {head['beam']['topk']}"
+                      + (" (autotuned: fastest L within 0.3 pt of the FP32 "
+                         "build's recall@10 on separate tuning queries)"
+                         if tune_beam else "")
+                      if head_prec != "fp32" else ""),
                    "n_base": n, "dim": dim, "M": M, "L": L,
+                   "screened_L": head["beam"]["L"],
                    "precision": head_prec},
         "roofline": head["roofline"],
         "cpu_baseline": cpu,

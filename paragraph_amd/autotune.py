@@ -7,7 +7,10 @@ Knobs (pg_params.tune_flags and table_size):
   - split-BF16 screen in the screened prune / merge on / off;
   - visited-table size (FP32: exact set, any size above the bound gives the
     same rows; screened: visited-cache size, 2x / 4x / 8x (Km + Kp*Kd)).
-Beam knobs (Km, Kp, Kd, rerank) change results and are not tuned here.
+Beam knobs change results: tune_beam picks the beam width of a screened
+build (Km = L, Kp = ceil(L/16), topk = 2L re-rank width) as the fastest one
+whose recall@10 stays within a tolerance of the FP32 build's; the FP32 path
+keeps the reference mapping.
 
 Method: one default build gives a realistic graph; then the search stage is
 timed on a sample of base rows for every search knob set, and the prune
@@ -139,3 +142,56 @@ def apply(p, best):
     pp.tune_flags = (p.tune_flags & ~0xBF) | best["tune_flags"]
     pp.table_size = best["table_size"]
     return pp
+
+
+def beam_params(p, L):
+    """Copy of p with beam width L mapped like pg_params_init."""
+    pp = _native.PgParams.from_buffer_copy(p)
+    pp.Km = L
+    pp.Kp = (L + 15) // 16
+    pp.topk = 2 * L
+    pp.rerank = 0
+    pp.table_size = 0
+    return pp
+
+
+def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
+              margin_pt=0.2, reps=2, log=None):
+    """Screened builds: the fastest beam width L whose recall@10
+    (recall_fn(link_d), scored on tuning queries) is at least
+    ref_recall - (tol_pt - margin_pt) / 100.  Returns (params, report)."""
+    import torch
+    if p.precision == _native.PG_FP32:
+        raise ValueError("the FP32 path keeps the reference beam mapping")
+    if Ls is None:
+        lo = max(48, p.M)
+        Ls = list(range(p.Km, lo - 1, -16))
+    n = p.n_base
+    i32 = dict(dtype=torch.int32, device=base_d.device)
+    proj = torch.empty((n, p.proj_deg), **i32)
+    link = torch.empty((n, p.M), **i32)
+    floor = ref_recall - (tol_pt - margin_pt) / 100.0
+    report = {"ref_recall": ref_recall, "floor": floor, "tried": []}
+    best, best_t = None, float("inf")
+    t_start = time.perf_counter()
+    for L in Ls:
+        pp = beam_params(p, L)
+        if _native.check_params(pp)[0] != 0:
+            continue
+        with _native.Context(pp) as ctx:
+            t = _time(lambda: (ctx.stage_device(base_d, gt_d, proj, link),
+                               ctx.wait()), reps=reps)
+        rec = recall_fn(link)
+        ok = rec >= floor
+        report["tried"].append({"L": L, "ms": round(1000 * t, 2),
+                                "recall_at_10": round(rec, 4), "ok": ok})
+        if log:
+            log(f"autotune beam L={L}: {1000 * t:.1f} ms, recall@10 "
+                f"{rec:.4f} ({'ok' if ok else 'below floor'})")
+        if ok and t < best_t:
+            best, best_t = pp, t
+    report["seconds"] = round(time.perf_counter() - t_start, 2)
+    if best is None:
+        best = _native.PgParams.from_buffer_copy(p)
+    report["L"] = int(best.Km)
+    return best, report

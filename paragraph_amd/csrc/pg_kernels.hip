@@ -684,6 +684,7 @@ __device__ __forceinline__ bool visit_insert(uint32_t* tab, uint32_t tsize,
 template <int TQ, int U>
 struct EvalF32 {
   static constexpr bool kExactVisited = true;
+  static constexpr int kMinWaves = 1;  // occupancy target for the compiler
   static constexpr uint32_t kQueryLdsBytes = 0;
   bf16x8* qs;  // unused
   float qv[TQ];
@@ -740,6 +741,25 @@ struct EvalF32 {
   }
 };
 
+// Elements e0 .. e0+7 of a query row (zero past dim or when !on): two
+// 16-byte loads when the row start is 16-byte aligned and dim % 8 == 0,
+// else scalar loads.
+__device__ __forceinline__ void load_query_octet(const float* __restrict__ x,
+                                                 uint32_t dim, bool on,
+                                                 uint32_t e0, float (&v)[8]) {
+  if (on && (dim & 7u) == 0 && e0 + 8u <= dim &&
+      (reinterpret_cast<uintptr_t>(x) & 15u) == 0) {
+    const float4 a = *reinterpret_cast<const float4*>(x + e0);
+    const float4 b = *reinterpret_cast<const float4*>(x + e0 + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      v[j] = (on && e0 + j < dim) ? x[e0 + j] : 0.0f;
+  }
+}
+
 // EvalBF16: screening distances on MFMA.  The base is a BF16 copy with rows
 // zero-padded to dpad (a multiple of 32).  16 candidates form the A operand
 // of v_mfma_f32_16x16x32_bf16 (lane l holds row l%16, elements
@@ -749,6 +769,9 @@ struct EvalF32 {
 template <int NCH, int G>
 struct EvalBF16 {
   static constexpr bool kExactVisited = false;
+  // 5 waves per SIMD (<= 96 VGPRs) measured faster for one group of <= 256
+  // padded elements than 4 waves without register spills
+  static constexpr int kMinWaves = (NCH <= 8 && G == 1) ? 5 : 1;
   static constexpr uint32_t kQueryLdsBytes = 0;
   bf16x8* qs;  // unused
   bf16x8 qf[NCH];  // lane l: query elements 32c + 8(l/16) .. +7 of chunk c
@@ -758,15 +781,13 @@ struct EvalBF16 {
   __device__ __forceinline__ void load_query(const float* __restrict__ queries,
                                              uint32_t q) {
     const uint32_t lane = lane_id();
+    const float* x = queries + size_t(q) * dim;
+    float v[8];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
+      load_query_octet(x, dim, uint32_t(c) < nch, 32u * c + 8u * (lane >> 4), v);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const uint32_t e = 32u * c + 8u * (lane >> 4) + j;
-        qf[c][j] = (__bf16)((uint32_t(c) < nch && e < dim)
-                                ? queries[size_t(q) * dim + e]
-                                : 0.0f);
-      }
+      for (int j = 0; j < 8; ++j) qf[c][j] = (__bf16)v[j];
     }
   }
 
@@ -834,6 +855,7 @@ __device__ __forceinline__ long pack_fp8x8(const float* v) {
 template <int NCH, int G>
 struct EvalFP8 {
   static constexpr bool kExactVisited = false;
+  static constexpr int kMinWaves = (NCH <= 8 && G == 1) ? 5 : 1;
   static constexpr uint32_t kQueryLdsBytes = 0;
   bf16x8* qs;  // unused
   long qf[NCH];
@@ -855,11 +877,9 @@ struct EvalFP8 {
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
       float v[8];
+      load_query_octet(x, dim, uint32_t(c) < nch, 32u * c + 8u * (lane >> 4), v);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const uint32_t e = 32u * c + 8u * (lane >> 4) + j;
-        v[j] = (uint32_t(c) < nch && e < dim) ? x[e] * inv : 0.0f;
-      }
+      for (int j = 0; j < 8; ++j) v[j] *= inv;
       qf[c] = pack_fp8x8(v);
     }
   }
@@ -967,10 +987,97 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
   return false;
 }
 
-// W: list width in 64-entry slices per lane for the merge (Km and Kp*Kd
-// both <= 64*W); smaller W keeps fewer registers live.
-template <class Eval, int W>
-__global__ __launch_bounds__(64) void search_kernel(
+// Merge by ranks, for any number of entering misses.  The misses below the
+// current Km-th key T enter (keys are unique per id); they are compacted into
+// ebuf, each one's position is #(entering < e) + #(kept < e) (a scan of ebuf
+// plus a binary search, one lane per entering key), and they are placed into
+// a staging copy of the top Km held in the dead tail region (it must hold Km
+// keys).  The kept keys then fill the free slots in order: free slot p takes
+// old key p - #(occupied slots before p).  Same list as merge_misses.
+// Returns 1 when this iteration is the last one (nothing enters and no
+// unexpanded key is in the top Km, or force_final), else 0.
+template <int CH, int HPL>
+__device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
+                            const uint64_t* tail, const uint32_t* todo,
+                            uint32_t m, uint64_t* ebuf, bool force_final) {
+  const uint32_t lane = lane_id();
+  const uint64_t T = keys[Km - 1];
+  uint64_t me[CH];
+  uint32_t slot[CH];
+  bool in_e[CH];
+  uint32_t ne = 0;
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    me[c] = kPadKey;
+    in_e[c] = false;
+    slot[c] = 0;
+    if (64u * c < m) {
+      const uint32_t idx = lane + 64u * c;
+      if (idx < m) me[c] = tail[todo[idx]];
+      in_e[c] = me[c] < T;
+      const uint64_t em = __ballot(in_e[c]);
+      slot[c] = ne + __popcll(em & lanemask_lt());
+      ne += __popcll(em);
+    }
+  }
+  bool unexp = false;
+#pragma unroll
+  for (int s = 0; s < HPL; ++s) {
+    const uint32_t i = lane + 64u * s;
+    if (i < Km && (keys[i] & 1ull) == 0) unexp = true;
+  }
+  if (ne == 0) return (force_final || __ballot(unexp) == 0) ? 1 : 0;
+  if (force_final) return 1;
+  __syncthreads();  // ebuf overlays the todo list read above
+#pragma unroll
+  for (int c = 0; c < CH; ++c)
+    if (in_e[c]) ebuf[slot[c]] = me[c];
+#pragma unroll
+  for (int s = 0; s < HPL; ++s) {
+    const uint32_t p = lane + 64u * s;
+    if (p < Km) stage[p] = kPadKey;
+  }
+  __syncthreads();
+  for (uint32_t e0 = 0; e0 < ne; e0 += kWave) {
+    const bool own = e0 + lane < ne;
+    const uint64_t e = own ? ebuf[e0 + lane] : kPadKey;
+    uint32_t r = 0;
+#pragma unroll 4
+    for (uint32_t j = 0; j < ne; ++j) r += ebuf[j] < e ? 1u : 0u;
+    if (own) {
+      r += count_less(keys, Km, e);
+      if (r < Km) stage[r] = e;
+    }
+  }
+  __syncthreads();
+  uint64_t v[HPL];
+  uint32_t occ_before = 0;
+#pragma unroll
+  for (int s = 0; s < HPL; ++s) {
+    const uint32_t p = lane + 64u * s;
+    v[s] = p < Km ? stage[p] : kPadKey;
+    const bool occ = p < Km && v[s] != kPadKey;
+    const uint64_t om = __ballot(occ);
+    const uint32_t before = occ_before + __popcll(om & lanemask_lt());
+    if (p < Km && !occ) v[s] = keys[p - before];
+    occ_before += __popcll(om);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < HPL; ++s) {
+    const uint32_t p = lane + 64u * s;
+    if (p < Km) keys[p] = v[s];
+  }
+  __syncthreads();
+  return 0;
+}
+
+// HPL: top-list slices of 64 keys per lane (Km <= 64*HPL), CH: candidate
+// slices (Kp*Kd <= 64*CH).  Compile-time slice counts keep the merge free of
+// dead slices; larger lists fall back to a full sort per iteration.
+template <class Eval, int HPL, int CH>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     Eval ev, const float* __restrict__ queries, uint32_t n_query,
     uint32_t n_base,
     const uint32_t* __restrict__ graph, uint32_t gdeg, uint32_t Km,
@@ -999,7 +1106,9 @@ __global__ __launch_bounds__(64) void search_kernel(
   };
   uint32_t q = queue ? next_query() : blockIdx.x;
   while (q < n_query) {
-  const bool fast_merge = Km <= 64u * W && kpkd <= 64u * W;
+  const bool fast_merge = Km <= 64u * HPL && kpkd <= 64u * CH;
+  // the rank merge stages the new top list in the tail region
+  const bool rank_merge = fast_merge && lp2 - Km >= Km;
   ev.load_query(queries, q);
   // initial list: [0, Km) = (FLT_MAX, tomb); [Km, Km+kpkd) = entry points
   for (uint32_t i = lane; i < lp2; i += kWave)
@@ -1092,16 +1201,19 @@ __global__ __launch_bounds__(64) void search_kernel(
     __syncthreads();
     const bool last_allowed = total_iter + 1 >= max_iters;
     bool final_sort = true;
-    if (fast_merge) {
+    if (rank_merge) {
+      final_sort = merge_ranked<CH, HPL>(keys, Km, tail, tail, todo, ntodo, ms,
+                                         last_allowed) == 1;
+    } else if (fast_merge) {
       if (ntodo <= 64)
-        final_sort = merge_misses<1, W>(keys, Km, tail, todo, ntodo, ms,
+        final_sort = merge_misses<1, HPL>(keys, Km, tail, todo, ntodo, ms,
                                         last_allowed);
-      else if (W <= 2 || ntodo <= 128)
-        final_sort = merge_misses<(W < 2 ? W : 2), W>(keys, Km, tail, todo,
+      else if (CH <= 2 || ntodo <= 128)
+        final_sort = merge_misses<(CH < 2 ? CH : 2), HPL>(keys, Km, tail, todo,
                                                       ntodo, ms, last_allowed);
-      else
-        final_sort = merge_misses<W, W>(keys, Km, tail, todo, ntodo, ms,
-                                        last_allowed);
+      else  // the register bitonic sort needs a power-of-two width
+        final_sort = merge_misses<(CH <= 4 ? 4 : 8), HPL>(
+            keys, Km, tail, todo, ntodo, ms, last_allowed);
     }
     if (final_sort) bitonic_sort(keys, lp2);
     ++total_iter;
@@ -1292,7 +1404,15 @@ static hipError_t launch_search_eval(
   uint32_t kd_shift = 0xFFu;
   for (uint32_t sh = 0; sh < 31; ++sh)
     if ((1u << sh) == Kd) kd_shift = sh;
-  const void* fn = reinterpret_cast<const void*>(&search_kernel<Eval, 4>);
+  const uint32_t kpkd = Kp * Kd;
+  // top-list / candidate slices per lane: 2/2 up to 128 entries each, 3/3
+  // up to 192, else 4/4 (lists beyond 256 sort fully every iteration)
+  const int slices = (Km <= 128 && kpkd <= 128) ? 2
+                     : (Km <= 192 && kpkd <= 192) ? 3 : 4;
+  const void* fn =
+      slices == 2 ? reinterpret_cast<const void*>(&search_kernel<Eval, 2, 2>)
+      : slices == 3 ? reinterpret_cast<const void*>(&search_kernel<Eval, 3, 3>)
+                    : reinterpret_cast<const void*>(&search_kernel<Eval, 4, 4>);
   (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                             int(smem));
   uint32_t grid = n_query;
@@ -1310,9 +1430,14 @@ static hipError_t launch_search_eval(
     hipError_t e = launch_fill_u32(queue, 1, 0u, s);
     if (e != hipSuccess) return e;
   }
-  search_kernel<Eval, 4><<<grid, 64, smem, s>>>(
-      ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,
-      entry_seed, max_iters, table_size, lp2, kd_shift, out, stats, queue);
+#define PG_SEARCH_LAUNCH(H, C)                                              \
+  search_kernel<Eval, H, C><<<grid, 64, smem, s>>>(                         \
+      ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter, \
+      entry_seed, max_iters, table_size, lp2, kd_shift, out, stats, queue)
+  if (slices == 2) PG_SEARCH_LAUNCH(2, 2);
+  else if (slices == 3) PG_SEARCH_LAUNCH(3, 3);
+  else PG_SEARCH_LAUNCH(4, 4);
+#undef PG_SEARCH_LAUNCH
   return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@ Generator (numpy default_rng; seeds fixed here):
   offset   o ~ N(0, 0.5^2)^dim, drawn once                      seed 14
   training queries y = R (C_k + 0.3 N(0, I)) + o                seed 15
   held-out eval queries: same distribution                      seed 16
+  autotuner queries: same distribution                          seed 18
 Optional: L2 normalisation (base, and queries after rotation + offset), Zipf
 component choice for queries, ragged gt lengths padded with -1.
 
@@ -161,4 +162,13 @@ def make_config(name, scale=1.0, n_eval=0, device=None, with_gt=True):
                       gt_len_range=c.get("gt_len_range"), n_eval=n_eval,
                       device=device, with_gt=with_gt)
     w["M"], w["L"] = c["M"], c["L"]
+    w["cfg"] = c
     return w
+
+
+def tune_queries(name, n):
+    """Queries the autotuner scores recall on (disjoint seed from the
+    training and held-out eval queries)."""
+    c = CONFIGS[name]
+    return query_vectors(n, c["dim"], c["n_comp"], 18,
+                         c.get("normalize", False), c.get("zipf"))
