@@ -769,12 +769,13 @@ __device__ __forceinline__ void load_query_octet(const float* __restrict__ x,
 template <int NCH, int G>
 struct EvalBF16 {
   static constexpr bool kExactVisited = false;
-  // 5 waves per SIMD (<= 96 VGPRs) measured faster for one group of <= 256
-  // padded elements than 4 waves without register spills
+  // 5 waves per SIMD (<= 96 VGPRs) for one group of <= 256 padded elements
   static constexpr int kMinWaves = (NCH <= 8 && G == 1) ? 5 : 1;
-  static constexpr uint32_t kQueryLdsBytes = 0;
-  bf16x8* qs;  // unused
-  bf16x8 qf[NCH];  // lane l: query elements 32c + 8(l/16) .. +7 of chunk c
+  // The query's B-operand fragments live in LDS (qs[4c + k] = elements
+  // 32c + 8k .. +7), read per MFMA: replicating them in registers would
+  // cost 4*NCH VGPRs per lane.
+  static constexpr uint32_t kQueryLdsBytes = NCH * 4 * 16;
+  bf16x8* qs;
   const __bf16* __restrict__ base16;
   uint32_t dim, dpad, nch;
 
@@ -782,12 +783,14 @@ struct EvalBF16 {
                                              uint32_t q) {
     const uint32_t lane = lane_id();
     const float* x = queries + size_t(q) * dim;
-    float v[8];
+    if (lane < 4u * NCH) {
+      const uint32_t c = lane >> 2, k = lane & 3u;
+      float v[8];
+      load_query_octet(x, dim, c < nch, 32u * c + 8u * k, v);
+      bf16x8 f;
 #pragma unroll
-    for (int c = 0; c < NCH; ++c) {
-      load_query_octet(x, dim, uint32_t(c) < nch, 32u * c + 8u * (lane >> 4), v);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) qf[c][j] = (__bf16)v[j];
+      for (int j = 0; j < 8; ++j) f[j] = (__bf16)v[j];
+      qs[lane] = f;
     }
   }
 
@@ -818,10 +821,12 @@ struct EvalBF16 {
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        // keep the fragment reads inside the loop (not hoisted to registers)
+        __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g][c], qf[c], acc,
-                                                        0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g][c], qs[4 * c + kq],
+                                                        acc, 0, 0, 0);
         // lane l (l%16 < 4) writes candidate 4(l/16) + l%4
         if (m < 4u) {
           const uint32_t cm = 4u * kq + m;
@@ -856,9 +861,9 @@ template <int NCH, int G>
 struct EvalFP8 {
   static constexpr bool kExactVisited = false;
   static constexpr int kMinWaves = (NCH <= 8 && G == 1) ? 5 : 1;
-  static constexpr uint32_t kQueryLdsBytes = 0;
-  bf16x8* qs;  // unused
-  long qf[NCH];
+  // e4m3 query fragments in LDS: 8 bytes per (chunk, lane group)
+  static constexpr uint32_t kQueryLdsBytes = NCH * 4 * 8;
+  bf16x8* qs;
   float qscale;
   const uint8_t* __restrict__ base8;
   const float* __restrict__ scale8;
@@ -874,13 +879,13 @@ struct EvalFP8 {
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     qscale = mx > 0.0f ? mx / 448.0f : 1.0f;
     const float inv = 1.0f / qscale;
-#pragma unroll
-    for (int c = 0; c < NCH; ++c) {
+    if (lane < 4u * NCH) {
+      const uint32_t c = lane >> 2, k = lane & 3u;
       float v[8];
-      load_query_octet(x, dim, uint32_t(c) < nch, 32u * c + 8u * (lane >> 4), v);
+      load_query_octet(x, dim, c < nch, 32u * c + 8u * k, v);
 #pragma unroll
       for (int j = 0; j < 8; ++j) v[j] *= inv;
-      qf[c] = pack_fp8x8(v);
+      reinterpret_cast<long*>(qs)[lane] = pack_fp8x8(v);
     }
   }
 
@@ -905,10 +910,12 @@ struct EvalFP8 {
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const long* q8 = reinterpret_cast<const long*>(qs);
+        __asm__ volatile("" ::: "memory");
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[g][c], qf[c], acc,
-                                                           0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(
+              a[g][c], q8[4 * c + kq], acc, 0, 0, 0);
         if (m < 4u) {
           const uint32_t cm = 4u * kq + m;
           const uint32_t idx = g0 + 16u * g + cm;
@@ -1654,6 +1661,9 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(
   __shared__ uint32_t todo[kPairMaxK];
   __shared__ uint64_t tail[kPairMaxK];
   __shared__ uint32_t ntodo_s;
+  // query fragments of evaluators that keep them in LDS
+  __shared__ bf16x8 qbuf[Eval::kQueryLdsBytes ? Eval::kQueryLdsBytes / 16 : 1];
+  ev.qs = qbuf;
   const uint32_t q = blockIdx.x;
   const uint32_t lane = lane_id();
   if (lane == 0) ntodo_s = 0;
@@ -1675,6 +1685,7 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(
   }
   const uint32_t ntodo = ntodo_s;
   ev.load_query(queries, q);
+  __syncthreads();
   ev.eval(cand, todo, ntodo, tail);
   __syncthreads();
   for (uint32_t t = lane; t < k; t += kWave) {
