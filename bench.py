@@ -149,6 +149,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--no-dist-error", action="store_true")
+    ap.add_argument("--no-host-window", action="store_true",
+                    help="skip the one build from host buffers (PCIe-"
+                         "inclusive rate, reported beside value)")
     ap.add_argument("--n-eval", type=int, default=10000)
     ap.add_argument("--tune-flags", type=int, default=None,
                     help="perf-only knobs for the FP32 build (default: the "
@@ -323,6 +326,28 @@ def main():
         r["ctx"] = ctx
         results[prec] = r
 
+    # ---- host window: one build of the value path from pageable host
+    # buffers through the host entry point (H2D + stage + D2H); reported
+    # beside `value`, never as it
+    host_window = None
+    if not args.no_host_window:
+        hp = results[[x for x in precs if x != "fp32"][0] if any(
+            x != "fp32" for x in precs) else "fp32"]
+        hctx = hp["ctx"]
+        hproj = np.empty((n, hctx.params.proj_deg), np.uint32)
+        hlink = np.empty((n, M), np.uint32)
+        base_h = np.ascontiguousarray(w["base"])
+        torch.cuda.synchronize()
+        t_h = time.perf_counter()
+        hctx.stage_host(base_h, gt_u32, hproj, hlink)
+        hctx.wait()
+        ms_h = 1000.0 * (time.perf_counter() - t_h)
+        host_window = {"ms": round(ms_h, 2),
+                       "base_points_per_s": round(n / (ms_h / 1000.0), 1),
+                       "same_rows_as_device_entry": bool(np.array_equal(
+                           hlink, hp["link"].cpu().numpy().view(np.uint32)))}
+        log(f"host window (H2D + stage + D2H): {host_window}")
+
     # ---- CPU baseline: the oracle on a bounded node sample
     cpu = None
     if not args.no_cpu_baseline:
@@ -351,6 +376,8 @@ def main():
                        for k, v in r.items() if k not in ("link", "ctx")}
     if dist_err is not None:
         extra["distance_error_vs_fp64"] = dist_err
+    if host_window is not None:
+        extra["host_window"] = host_window
     for sp in screened:
         if "fp32" not in results:
             break

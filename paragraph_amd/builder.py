@@ -35,6 +35,12 @@ class Builder:
 
     def LaunchBuild(self, M=32, L=128, device=0, precision=_native.PG_FP32,
                     **overrides):
+        if isinstance(precision, str):
+            names = {"fp32": _native.PG_FP32, "bf16": _native.PG_BF16,
+                     "fp8": _native.PG_FP8E4M3}
+            if precision not in names:
+                raise ValueError(f"precision must be one of {sorted(names)}")
+            precision = names[precision]
         n, dim = self.base_.shape
         q, gt_k = self.gt_ids_.shape
         p = _native.make_params(n, dim, q, M, L, gt_k=gt_k,

@@ -55,8 +55,10 @@ struct pg_ctx {
   void* base8 = nullptr;    // FP8 path: per-row scaled e4m3 rows
   float* scale8 = nullptr;  // FP8 path: row scales
   float* norms = nullptr;   // screened paths: FP32 row norms
+  float* rnorms = nullptr;  // plain screen: |x - bf16(x)| per row
   uint32_t dpad = 0;        // BF16 row stride (dim rounded up to 32)
-  float eps_c = 0.0f;       // screening bound factor (plain or split)
+  float eps_c = 0.0f;       // split screen bound factor
+  float gam_a = 0.0f, gam_c = 0.0f;  // plain screen: MFMA / chain error
   unsigned long long* stats = nullptr;
   unsigned long long* stage_cnt = nullptr;  // (kMaxStages+1) x 16 snapshots
   uint32_t* queue = nullptr;                // persistent-search work counter
@@ -200,12 +202,14 @@ int record(pg_ctx* c, int idx) {
 pg::ScreenArgs screen_args(const pg_ctx* c) {
   return pg::ScreenArgs{reinterpret_cast<const __bf16*>(c->base16),
                         reinterpret_cast<const __bf16*>(c->base16lo), c->norms,
-                        c->dpad, c->eps_c};
+                        c->base16lo ? nullptr : c->rnorms, c->dpad, c->eps_c,
+                        c->gam_a, c->gam_c};
 }
 
 int prepare_screen(pg_ctx* c, const float* d_base) {
   PG_HIP(c, pg::launch_prepare_screen(d_base, c->p.n_base, c->p.dim, c->dpad,
                                       c->base16, c->base16lo, c->norms,
+                                      c->base16lo ? nullptr : c->rnorms,
                                       c->stream));
   if (c->base8)
     PG_HIP(c, pg::launch_prepare_fp8(d_base, c->p.n_base, c->p.dim, c->dpad,
@@ -431,6 +435,14 @@ pg_ctx* pg_create(int device, const pg_params* p) {
   c->dpad = (p->dim + 31) / 32 * 32;
   c->eps_c = (p->tune_flags & kTuneSplit) ? screen_eps_split(p->dim, c->dpad)
                                            : screen_eps(p->dim, c->dpad);
+  // plain screen, per-pair bound with residual norms: gamma factors of the
+  // MFMA accumulation over dpad terms (2^-23, any rounding of the adds) and
+  // of the exact chain over dim terms (2^-24)
+  {
+    const double ua = std::ldexp(1.0, -23), uc = std::ldexp(1.0, -24);
+    c->gam_a = float(c->dpad * ua / (1.0 - c->dpad * ua));
+    c->gam_c = float(p->dim * uc / (1.0 - p->dim * uc));
+  }
   auto fail = [&](int) {
     std::memcpy(g_err, c->err, sizeof(g_err));
     pg_destroy(c);
@@ -463,7 +475,8 @@ pg_ctx* pg_create(int device, const pg_params* p) {
       (p->precision != PG_FP32 &&
        (dev_alloc(c, reinterpret_cast<uint16_t**>(&c->base16),
                   N * ((p->dim + 31) / 32 * 32), "base_bf16") ||
-        dev_alloc(c, &c->norms, N, "norms"))) ||
+        dev_alloc(c, &c->norms, N, "norms") ||
+        dev_alloc(c, &c->rnorms, N, "rnorms"))) ||
       (p->precision != PG_FP32 && (p->tune_flags & kTuneSplit) &&
        dev_alloc(c, reinterpret_cast<uint16_t**>(&c->base16lo),
                  N * ((p->dim + 31) / 32 * 32), "base_bf16_lo")) ||
@@ -502,6 +515,7 @@ void pg_destroy(pg_ctx* c) {
   c->base16 = c->base8 = c->base16lo = nullptr;
   dev_free(c->scale8);
   dev_free(c->norms);
+  dev_free(c->rnorms);
   drop_graph(c);
   dev_free(c->stats);
   dev_free(c->stage_cnt);

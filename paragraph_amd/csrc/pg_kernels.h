@@ -10,9 +10,11 @@ namespace pg {
 struct ScreenArgs {
   const __bf16* base16;    // bf16(x), rows padded to dpad, zero-filled
   const __bf16* base16lo;  // bf16(x - hi) for the split screen, or nullptr
-  const float* norms;      // FP32 row norms
+  const float* norms;      // FP32 row norms |x|
+  const float* rnorms;     // |x - bf16(x)| per row (plain screen), or nullptr
   uint32_t dpad;
-  float eps_c;             // |screened - exact chain| <= eps_c * |x| * |y|
+  float eps_c;             // split screen: |screened - chain| <= eps_c |x||y|
+  float gam_a, gam_c;      // plain screen: accumulation / chain factors
 };
 
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
@@ -43,7 +45,7 @@ size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
                          uint32_t Kp);
 hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
                                  uint32_t dpad, void* out16, void* lo16,
-                                 float* norms, hipStream_t s);
+                                 float* norms, float* rnorms, hipStream_t s);
 // base16 == nullptr: exact FP32 search; else BF16-screened search over the
 // padded BF16 copy (row stride dpad elements).  queue != nullptr: persistent
 // grid fed by the device counter *queue (reset by the launcher).

@@ -148,9 +148,9 @@ __device__ uint32_t rng_select(const float* __restrict__ base, uint32_t n_base,
 // chains give the block against each 16-wide tile of kept entries and
 // against the block itself (BF16 rows, or the hi + lo split of each row for a
 // ~30x tighter bound; FP32 accumulation).  A screened value s decides a
-// comparison only when the bound |s - chain| <= eps_c*|c|*|k| (rounding of
-// the rows, FP32 accumulation, the chain's own error, Cauchy-Schwarz) puts
-// both sides of d_i out of reach;
+// comparison only when the bound on |s - chain| (pair_bound: rounding of the
+// rows, FP32 accumulation, the chain's own error) puts both sides of d_i out
+// of reach;
 // otherwise the exact FP32 chain is computed (counted as a fallback).  So
 // every decision equals the plain FP32 path's.
 // 1: occluded for sure, 0: not occluded for sure, -1: undecided.
@@ -160,11 +160,25 @@ __device__ __forceinline__ int screen_cmp(float s, float d, float e) {
   return -1;
 }
 
+// Bound on |screened - exact chain| for the pair (x, y), with n = |x| and
+// r = |x - bf16(x)| per row.  Plain screen: x.y - x^.y^ = x^.r_y + r_x.y^ +
+// r_x.r_y, at most h_x r_y + r_x h_y + r_x r_y with h = n + r >= |x^|
+// (Cauchy-Schwarz); plus the MFMA accumulation gam_a h_x h_y (products of
+// BF16 values are exact in FP32) and the chain's own gam_c n_x n_y; 1% margin
+// for the FP32 norms and this evaluation.  Split screen: eps_c n_x n_y.
+__device__ __forceinline__ float pair_bound(const ScreenArgs& sc, float nx,
+                                            float rx, float ny, float ry) {
+  if (sc.rnorms == nullptr) return sc.eps_c * nx * ny;
+  const float hx = nx + rx, hy = ny + ry;
+  return 1.01f * (sc.gam_a * hx * hy + hx * ry + rx * hy + rx * ry +
+                  sc.gam_c * nx * ny);
+}
+
 template <bool kStopIds, int NCH, int NKT, bool kSplit>
 __device__ uint32_t rng_select_screened(
     const float* __restrict__ base, const ScreenArgs& sc, uint32_t n_base,
     uint32_t dim, const uint64_t* skey, const uint32_t* sid, uint32_t n_scan,
-    uint32_t cap, uint32_t* kept, float* kept_norm, uint32_t nk, float* gs,
+    uint32_t cap, uint32_t* kept, float2* kept_nr, uint32_t nk, float* gs,
     unsigned long long* pair_total, unsigned long long* fb_total) {
   constexpr uint32_t GW = NKT * 16 + 16;  // gs row width
   const uint32_t lane = lane_id();
@@ -193,96 +207,115 @@ __device__ uint32_t rng_select_screened(
     const bool active = in && m < lim;
     const uint32_t nk0 = nk;
     // ---- Gram tiles
-    // Gram tiles.  Plain: BF16 rows.  Split (kSplit): rows x = hi + lo as two
-    // BF16 copies and x.y taken as hi.hi + lo.hi + hi.lo (three MFMAs).
-    bf16x8 ah[NCH], al[kSplit ? NCH : 1];
-    {
-      const size_t off = size_t(active ? my_id : 0u) * sc.dpad + 8u * kq;
-      const bf16x8* rh = reinterpret_cast<const bf16x8*>(sc.base16 + off);
-      const bf16x8* rl = reinterpret_cast<const bf16x8*>(
-          kSplit ? sc.base16lo + off : sc.base16 + off);
+    // Plain: BF16 rows.  Split (kSplit): rows x = hi + lo as two BF16 copies
+    // and x.y taken as hi.hi + lo.hi + hi.lo (three MFMAs).  The chunks are
+    // taken in register groups of at most 8: the block's fragments of one
+    // group stay in registers while every kept tile accumulates.
+    constexpr int CG = NCH < 8 ? NCH : 8;
+    f32x4 acc_t[NKT], acc_s = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        const bool ld = active && uint32_t(c) < nch;
-        if (ld) {
-          ah[c] = rh[4 * c];
-        } else {
+    for (int t = 0; t < NKT; ++t) acc_t[t] = acc_s;
+    const size_t own_off = size_t(active ? my_id : 0u) * sc.dpad + 8u * kq;
 #pragma unroll
-          for (int j = 0; j < 8; ++j) ah[c][j] = (__bf16)0.0f;
-        }
-        if constexpr (kSplit) {
+    for (int g0 = 0; g0 < NCH; g0 += CG) {
+      bf16x8 ah[CG], al[kSplit ? CG : 1];
+      {
+        const bf16x8* rh = reinterpret_cast<const bf16x8*>(sc.base16 + own_off);
+        const bf16x8* rl = reinterpret_cast<const bf16x8*>(
+            kSplit ? sc.base16lo + own_off : sc.base16 + own_off);
+#pragma unroll
+        for (int c = 0; c < CG; ++c) {
+          const bool ld = active && uint32_t(g0 + c) < nch;
           if (ld) {
-            al[c] = rl[4 * c];
+            ah[c] = rh[4 * (g0 + c)];
           } else {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) al[c][j] = (__bf16)0.0f;
+            for (int j = 0; j < 8; ++j) ah[c][j] = (__bf16)0.0f;
           }
-        }
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < NKT; ++t) {
-      if (16u * t < nk0) {
-        const uint32_t kidx = 16u * t + m;
-        const bool kv = kidx < nk0;
-        const size_t off = size_t(kv ? kept[kidx] : 0u) * sc.dpad + 8u * kq;
-        const bf16x8* bh = reinterpret_cast<const bf16x8*>(sc.base16 + off);
-        const bf16x8* bl = reinterpret_cast<const bf16x8*>(
-            kSplit ? sc.base16lo + off : sc.base16 + off);
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int c = 0; c < NCH; ++c) {
-          const bool ld = kv && uint32_t(c) < nch;
-          bf16x8 h;
-          if (ld) {
-            h = bh[4 * c];
-          } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) h[j] = (__bf16)0.0f;
-          }
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], h, acc, 0, 0, 0);
           if constexpr (kSplit) {
-            bf16x8 lw;
             if (ld) {
-              lw = bl[4 * c];
+              al[c] = rl[4 * (g0 + c)];
             } else {
 #pragma unroll
-              for (int j = 0; j < 8; ++j) lw[j] = (__bf16)0.0f;
+              for (int j = 0; j < 8; ++j) al[c][j] = (__bf16)0.0f;
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], h, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], lw, acc, 0, 0, 0);
           }
         }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) gs[(4u * kq + j) * GW + 16u * t + m] = acc[j];
       }
-    }
-    {
-      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], ah[c], acc, 0, 0, 0);
-        if constexpr (kSplit) {
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], ah[c], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], al[c], acc, 0, 0, 0);
+      for (int t = 0; t < NKT; ++t) {
+        if (16u * t < nk0) {
+          const uint32_t kidx = 16u * t + m;
+          const bool kv = kidx < nk0;
+          const size_t off = size_t(kv ? kept[kidx] : 0u) * sc.dpad + 8u * kq;
+          const bf16x8* bh = reinterpret_cast<const bf16x8*>(sc.base16 + off);
+          const bf16x8* bl = reinterpret_cast<const bf16x8*>(
+              kSplit ? sc.base16lo + off : sc.base16 + off);
+#pragma unroll
+          for (int c = 0; c < CG; ++c) {
+            const bool ld = kv && uint32_t(g0 + c) < nch;
+            bf16x8 h;
+            if (ld) {
+              h = bh[4 * (g0 + c)];
+            } else {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) h[j] = (__bf16)0.0f;
+            }
+            acc_t[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], h,
+                                                               acc_t[t], 0, 0,
+                                                               0);
+            if constexpr (kSplit) {
+              bf16x8 lw;
+              if (ld) {
+                lw = bl[4 * (g0 + c)];
+              } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) lw[j] = (__bf16)0.0f;
+              }
+              acc_t[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  al[c], h, acc_t[t], 0, 0, 0);
+              acc_t[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  ah[c], lw, acc_t[t], 0, 0, 0);
+            }
+          }
         }
       }
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        gs[(4u * kq + j) * GW + NKT * 16u + m] = acc[j];
+      for (int c = 0; c < CG; ++c) {
+        acc_s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], ah[c], acc_s, 0,
+                                                        0, 0);
+        if constexpr (kSplit) {
+          acc_s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[c], ah[c], acc_s,
+                                                          0, 0, 0);
+          acc_s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[c], al[c], acc_s,
+                                                          0, 0, 0);
+        }
+      }
     }
+#pragma unroll
+    for (int t = 0; t < NKT; ++t)
+      if (16u * t < nk0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          gs[(4u * kq + j) * GW + 16u * t + m] = acc_t[t][j];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      gs[(4u * kq + j) * GW + NKT * 16u + m] = acc_s[j];
     __syncthreads();
-    // ---- resolution: lane i < 16 owns block candidate i
+    // ---- resolution: lane i < 16 owns block candidate i (the first
+    // occluder in kept order ends a candidate's scan early)
     const bool own = lane < 16 && active;
     const float my_norm = own ? sc.norms[my_id] : 0.0f;
+    const float my_rn = (own && sc.rnorms) ? sc.rnorms[my_id] : 0.0f;
     const float* own_row = base + size_t(own ? my_id : 0u) * dim;
     bool occl = false;
     uint32_t my_pairs = 0, my_fb = 0;
     if (own) {
       for (uint32_t k = 0; k < nk0; ++k) {
         ++my_pairs;
+        const float2 knr = kept_nr[k];
         int dec = screen_cmp(-gs[m * GW + k], my_d,
-                             sc.eps_c * my_norm * kept_norm[k]);
+                             pair_bound(sc, my_norm, my_rn, knr.x, knr.y));
         if (dec < 0) {
           ++my_fb;
           dec = chain_dist(base + size_t(kept[k]) * dim, own_row, dim) < my_d;
@@ -300,9 +333,10 @@ __device__ uint32_t rng_select_screened(
       const int j = __ffsll((long long)msk) - 1;
       const uint32_t kid = __shfl(my_id, j, 64);
       const float kn = __shfl(my_norm, j, 64);
+      const float kr = __shfl(my_rn, j, 64);
       if (lane == 0) {
         kept[nk] = kid;
-        kept_norm[nk] = kn;
+        kept_nr[nk] = make_float2(kn, kr);
       }
       ++nk;
       last = j;
@@ -313,7 +347,7 @@ __device__ uint32_t rng_select_screened(
       if (own && !occl && int(lane) > j) {
         ++my_pairs;
         int dec = screen_cmp(-gs[m * GW + NKT * 16u + j], my_d,
-                             sc.eps_c * my_norm * kn);
+                             pair_bound(sc, my_norm, my_rn, kn, kr));
         if (dec < 0) {
           ++my_fb;
           dec = chain_dist(base + size_t(kid) * dim, own_row, dim) < my_d;
@@ -352,7 +386,7 @@ __global__ __launch_bounds__(64) void prune_kernel(
   __shared__ uint64_t skey[DP2];
   __shared__ uint32_t sid[DP2];
   __shared__ uint32_t kept[128];
-  __shared__ float kept_norm[NCH ? 128 : 1];
+  __shared__ float2 kept_nr[NCH ? 128 : 1];
   __shared__ float gs[NCH ? 16 * (NKT * 16 + 16) : 1];
   const uint32_t b = blockIdx.x;
   const uint32_t lane = lane_id();
@@ -394,7 +428,9 @@ __global__ __launch_bounds__(64) void prune_kernel(
   if (!is_stop_id(id0, n_base)) {
     if (lane == 0) {
       kept[0] = id0;
-      if (NCH) kept_norm[0] = sc.norms[id0];
+      if (NCH)
+        kept_nr[0] = make_float2(sc.norms[id0],
+                                 sc.rnorms ? sc.rnorms[id0] : 0.0f);
     }
     __syncthreads();
     if constexpr (NCH == 0)
@@ -402,7 +438,7 @@ __global__ __launch_bounds__(64) void prune_kernel(
                             &pairs);
     else
       nk = rng_select_screened<true, NCH, NKT, kSplit>(base, sc, n_base, dim, skey,
-                                               sid, D, mp, kept, kept_norm, 1,
+                                               sid, D, mp, kept, kept_nr, 1,
                                                gs, &pairs, &fbk);
   }
   for (uint32_t j = lane; j < nk; j += kWave) pruned[size_t(b) * mp + j] = kept[j];
@@ -548,7 +584,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
   __shared__ uint64_t skey[128];
   __shared__ uint32_t sid[128];
   __shared__ uint32_t kept[128];
-  __shared__ float kept_norm[NCH ? 128 : 1];
+  __shared__ float2 kept_nr[NCH ? 128 : 1];
   __shared__ float gs[NCH ? 16 * (NKT * 16 + 16) : 1];
   __shared__ uint8_t added[128];
   const uint32_t c = blockIdx.x;
@@ -599,7 +635,9 @@ __global__ __launch_bounds__(64) void merge_kernel(
   unsigned long long pairs = 0, fbk = 0;
   if (lane == 0) {
     kept[0] = sid[uint32_t(skey[0])];
-    if (NCH) kept_norm[0] = sc.norms[kept[0]];
+    if (NCH)
+      kept_nr[0] = make_float2(sc.norms[kept[0]],
+                               sc.rnorms ? sc.rnorms[kept[0]] : 0.0f);
   }
   __syncthreads();
   uint32_t nk;
@@ -608,7 +646,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
                            &pairs);
   else
     nk = rng_select_screened<false, NCH, NKT, kSplit>(base, sc, n_base, dim, skey, sid,
-                                              n, deg, kept, kept_norm, 1, gs,
+                                              n, deg, kept, kept_nr, 1, gs,
                                               &pairs, &fbk);
   if (!strict && nk < deg) {
     // mark the first list slot equal to each kept id, then append the
@@ -1289,7 +1327,7 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
                         hipStream_t s) {
   if (n_base == 0) return hipSuccess;
   const ScreenArgs scv =
-      sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, 0, 0.0f};
+      sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, nullptr, 0, 0.0f, 0.0f, 0.0f};
 #define PG_PRUNE(DP, NC, NK, SP)                                           \
   prune_kernel<DP, NC, NK, SP><<<n_base, 64, 0, s>>>(                      \
       base, n_base, dim, rows, D, gt, winner, gt_k, mp, pruned, pcnt, stats, \
@@ -1359,7 +1397,7 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
                         const ScreenArgs* sc, hipStream_t s) {
   if (n_base == 0) return hipSuccess;
   const ScreenArgs scv =
-      sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, 0, 0.0f};
+      sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, nullptr, 0, 0.0f, 0.0f, 0.0f};
 #define PG_MERGE(NC, NK, SP)                                                 \
   merge_kernel<NC, NK, SP><<<n_base, 64, 0, s>>>(base, n_base, dim, off,     \
                                                  srcs, cap, pruned, pcnt, mp, \
@@ -1573,22 +1611,28 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
 __global__ __launch_bounds__(256) void prepare_screen_kernel(
     const float* __restrict__ base, uint32_t n, uint32_t dim, uint32_t dpad,
     __bf16* __restrict__ out16, __bf16* __restrict__ lo16,
-    float* __restrict__ norms) {
+    float* __restrict__ norms, float* __restrict__ rnorms) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wpb = blockDim.x / 64u;
   for (uint32_t r = blockIdx.x * wpb + threadIdx.x / 64u; r < n;
        r += gridDim.x * wpb) {
     const float* x = base + size_t(r) * dim;
-    float ss = 0.0f;
+    float ss = 0.0f, rr = 0.0f;
     for (uint32_t c = lane; c < dpad; c += 64u) {
       const float v = c < dim ? x[c] : 0.0f;
       const __bf16 h = (__bf16)v;
       out16[size_t(r) * dpad + c] = h;
-      if (lo16) lo16[size_t(r) * dpad + c] = (__bf16)(v - (float)h);
+      const float res = v - (float)h;  // exact in FP32
+      if (lo16) lo16[size_t(r) * dpad + c] = (__bf16)res;
       ss = __builtin_fmaf(v, v, ss);
+      rr = __builtin_fmaf(res, res, rr);
     }
     ss = wave_sum(ss);
-    if (lane == 0) norms[r] = __builtin_sqrtf(ss);
+    rr = wave_sum(rr);
+    if (lane == 0) {
+      norms[r] = __builtin_sqrtf(ss);
+      if (rnorms) rnorms[r] = __builtin_sqrtf(rr);
+    }
   }
 }
 
@@ -1632,12 +1676,12 @@ hipError_t launch_prepare_fp8(const float* base, uint32_t n, uint32_t dim,
 
 hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
                                  uint32_t dpad, void* out16, void* lo16,
-                                 float* norms, hipStream_t s) {
+                                 float* norms, float* rnorms, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint32_t blocks = (n + 3) / 4 < 65536u ? (n + 3) / 4 : 65536u;
   prepare_screen_kernel<<<blocks, 256, 0, s>>>(
       base, n, dim, dpad, reinterpret_cast<__bf16*>(out16),
-      reinterpret_cast<__bf16*>(lo16), norms);
+      reinterpret_cast<__bf16*>(lo16), norms, rnorms);
   return hipGetLastError();
 }
 

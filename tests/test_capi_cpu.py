@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from paragraph_amd import _native
@@ -99,3 +100,10 @@ def test_autotune_candidates_valid():
                 assert getattr(pp, f) == getattr(p, f)
         pp = autotune.apply(p, {"tune_flags": 0xA4, "table_size": 0})
         assert pp.tune_flags == 0xA4
+
+
+def test_builder_rejects_unknown_precision():
+    from paragraph_amd import Builder
+    b = Builder(np.zeros((4, 8), np.float32), np.zeros((4, 128), np.int32))
+    with pytest.raises(ValueError):
+        b.LaunchBuild(M=16, L=16, precision="fp16")

@@ -1,0 +1,38 @@
+"""The Python front end of the reference's Builder interface
+(include/Builder.hpp:6-41) on the GPU: host entry point, same rows as the
+golden fixture, screened builds selected by name."""
+import numpy as np
+import pytest
+
+from paragraph_amd import Builder
+from tests.golden import golden_io
+
+pytestmark = pytest.mark.gpu
+
+
+def test_builder_matches_golden(cuda):
+    g = golden_io.load_stage("stage_d200")
+    pr = g["params"]
+    b = Builder(g["base"], g["gt"].view(np.int32))
+    proj, link = b.LaunchBuild(M=pr["M"], L=pr["Km"])
+    assert np.array_equal(proj, g["proj"])
+    assert np.array_equal(link, g["link"])
+    assert b.stats_["search_iters"] == g["stats"]["search_iters"]
+
+
+def test_builder_screened_by_name(cuda):
+    g = golden_io.load_stage("stage_d200")
+    pr = g["params"]
+    n = pr["n_base"]
+    b = Builder(g["base"].ravel(), g["gt"].ravel(), dim=pr["dim"])
+    for prec in ("bf16", "fp8"):
+        proj, link = b.LaunchBuild(M=pr["M"], L=pr["Km"], precision=prec)
+        assert np.array_equal(proj, g["proj"])  # projection is FP32 work
+        valid = link != golden_io.TOMB
+        assert (link[valid] < n).all()
+        assert not (link == np.arange(n, dtype=np.uint32)[:, None]).any()
+        overlap = np.mean([len(set(a[a != golden_io.TOMB]) &
+                               set(c[c != golden_io.TOMB])) /
+                           max(1, (c != golden_io.TOMB).sum())
+                           for a, c in zip(link, g["link"])])
+        assert overlap > 0.8, (prec, overlap)
