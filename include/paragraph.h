@@ -14,7 +14,11 @@
  *  - Every call returns 0 on success or a negative pg_status; nothing
  *    calls exit().  pg_last_error() gives the message.
  *  - A context owns its HIP stream, its workspace and its output staging;
- *    one context per host thread.  Calls do not spawn CPU threads, so they
+ *    one context per host thread.  Work is enqueued on that stream only:
+ *    device buffers passed to a call (inputs, and outputs the caller
+ *    pre-filled) must be complete when the call is made (synchronize the
+ *    stream that produced them), and outputs are valid after pg_wait /
+ *    pg_sync.  Calls do not spawn CPU threads, so they
  *    may run beside an OpenMP CPU stage (tests/test_t2iend.cu:888-962).
  */
 #ifndef PARAGRAPH_H_

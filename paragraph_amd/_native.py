@@ -132,12 +132,18 @@ def _check(rc, ctx):
 
 
 def _dptr(t):
+    """Device pointer of a torch tensor.  The library works on its own
+    stream and requires device inputs (and output buffers a caller filled)
+    to be complete when a call is made, so the tensor's producing stream
+    (torch's current stream on its device) is synchronized first."""
     if t is None:
         return None
     if not t.is_cuda:
         raise PgError("device tensor expected")
     if not t.is_contiguous():
         raise PgError("contiguous tensor expected")
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
     return ctypes.c_void_p(t.data_ptr())
 
 

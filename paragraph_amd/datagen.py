@@ -124,6 +124,30 @@ def exact_topk_ip(queries, base, k, device=None, q_chunk=None):
     return out
 
 
+def subset_topk_ip(queries, base, k, subset, seed=19, device=None):
+    """Sharded top-k by inner product (ties to the lower id): the base is
+    split into random shards of about `subset` rows and query q takes the
+    exact top-k of shard q mod #shards.  Every base row can appear in gt,
+    at the cost of an N x subset product instead of N x N; a well-formed gt
+    for builds whose full exact gt is out of reach (it is only a build
+    input)."""
+    rng = np.random.default_rng(seed)
+    n = base.shape[0]
+    n_sh = max(1, -(-n // subset))
+    perm = rng.permutation(n)
+    shards = np.array_split(perm, n_sh)
+    nq = queries.shape[0]
+    out = np.full((nq, k), -1, np.int32)
+    for s, ids in enumerate(shards):
+        ids = np.sort(ids)
+        qsel = np.arange(s, nq, n_sh)
+        if qsel.size == 0:
+            continue
+        local = exact_topk_ip(queries[qsel], base[ids], k, device)
+        out[qsel] = np.where(local >= 0, ids[np.maximum(local, 0)], -1)
+    return out
+
+
 def ragged_gt(gt, lo, hi, seed=17):
     """Per-row length uniform in [lo, hi], rest padded with -1."""
     rng = np.random.default_rng(seed)
@@ -136,14 +160,19 @@ def ragged_gt(gt, lo, hi, seed=17):
 
 def make_workload(n_base, dim, n_comp, n_query=None, gt_k=128,
                   normalize=False, zipf=None, gt_len_range=None,
-                  n_eval=0, device=None, with_gt=True):
-    """Returns dict(base, queries, gt (int32, -1 padded), eval_queries)."""
+                  n_eval=0, device=None, with_gt=True, gt_subset=None):
+    """Returns dict(base, queries, gt (int32, -1 padded), eval_queries).
+    gt_subset: exact top-k within that many random base rows instead of all
+    (large builds)."""
     n_query = n_base if n_query is None else n_query
     base = base_vectors(n_base, dim, n_comp, normalize)
     queries = query_vectors(n_query, dim, n_comp, 15, normalize, zipf)
     w = dict(base=base, queries=queries)
     if with_gt:
-        gt = exact_topk_ip(queries, base, gt_k, device)
+        if gt_subset is not None and gt_subset < n_base:
+            gt = subset_topk_ip(queries, base, gt_k, gt_subset, device=device)
+        else:
+            gt = exact_topk_ip(queries, base, gt_k, device)
         if gt_len_range is not None:
             gt = ragged_gt(gt, *gt_len_range)
         w["gt"] = gt
@@ -153,14 +182,15 @@ def make_workload(n_base, dim, n_comp, n_query=None, gt_k=128,
     return w
 
 
-def make_config(name, scale=1.0, n_eval=0, device=None, with_gt=True):
+def make_config(name, scale=1.0, n_eval=0, device=None, with_gt=True,
+                gt_subset=None):
     """One of CONFIGS, optionally scaled down in n_base (for tests)."""
     c = dict(CONFIGS[name])
     n = max(64, int(c["n_base"] * scale))
     w = make_workload(n, c["dim"], c["n_comp"], normalize=c.get("normalize",
                       False), zipf=c.get("zipf"),
                       gt_len_range=c.get("gt_len_range"), n_eval=n_eval,
-                      device=device, with_gt=with_gt)
+                      device=device, with_gt=with_gt, gt_subset=gt_subset)
     w["M"], w["L"] = c["M"], c["L"]
     w["cfg"] = c
     return w
