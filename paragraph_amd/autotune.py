@@ -155,11 +155,19 @@ def beam_params(p, L):
     return pp
 
 
+def recall_floor(ref_recall, tol_pt=0.5, margin_pt=0.2, rel_tol=0.05):
+    """Lowest recall@10 the beam tuner accepts for a screened build."""
+    return ref_recall - min((tol_pt - margin_pt) / 100.0,
+                            rel_tol * ref_recall)
+
+
 def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
-              margin_pt=0.2, reps=2, log=None):
+              margin_pt=0.2, rel_tol=0.05, reps=2, log=None):
     """Screened builds: the fastest beam width L whose recall@10
     (recall_fn(link_d), scored on tuning queries) is at least
-    ref_recall - (tol_pt - margin_pt) / 100.  Returns (params, report)."""
+    ref_recall - min((tol_pt - margin_pt) / 100, rel_tol * ref_recall): an
+    absolute tolerance in points, capped at a relative loss for workloads
+    whose recall is itself small.  Returns (params, report)."""
     import torch
     if p.precision == _native.PG_FP32:
         raise ValueError("the FP32 path keeps the reference beam mapping")
@@ -170,7 +178,7 @@ def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
     i32 = dict(dtype=torch.int32, device=base_d.device)
     proj = torch.empty((n, p.proj_deg), **i32)
     link = torch.empty((n, p.M), **i32)
-    floor = ref_recall - (tol_pt - margin_pt) / 100.0
+    floor = recall_floor(ref_recall, tol_pt, margin_pt, rel_tol)
     report = {"ref_recall": ref_recall, "floor": floor, "tried": []}
     best, best_t = None, float("inf")
     t_start = time.perf_counter()

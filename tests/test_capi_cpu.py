@@ -107,3 +107,17 @@ def test_builder_rejects_unknown_precision():
     b = Builder(np.zeros((4, 8), np.float32), np.zeros((4, 128), np.int32))
     with pytest.raises(ValueError):
         b.LaunchBuild(M=16, L=16, precision="fp16")
+
+
+def test_beam_recall_floor():
+    from paragraph_amd import autotune
+    # absolute 0.3 pt when recall is large, 5% relative when it is small
+    assert abs(autotune.recall_floor(0.12) - 0.117) < 1e-12
+    assert abs(autotune.recall_floor(0.26) - 0.257) < 1e-12
+    assert abs(autotune.recall_floor(0.0066) - 0.0066 * 0.95) < 1e-12
+    for L in (128, 96, 64):
+        p = _native.make_params(1000, 64, 1000, 32, 128,
+                                precision=_native.PG_BF16)
+        pp = autotune.beam_params(p, L)
+        assert (pp.Km, pp.Kp, pp.topk) == (L, (L + 15) // 16, 2 * L)
+        assert _native.check_params(pp)[0] == 0
