@@ -87,14 +87,19 @@ typedef struct pg_params {
                            hipGraph once per buffer set and replay it (no
                            per-stage times in that mode); bit 7 (0x80):
                            split-BF16 screen in the screened prune / merge
-                           (tighter bound, fewer exact fallbacks) */
+                           (tighter bound, fewer exact fallbacks); bit 8
+                           (0x100): prune the link rows candidate by
+                           candidate (the reference's loop order) instead of
+                           kept-major sweeps (same rows) */
   uint32_t reserved[7];
 } pg_params;
 
 /* Counters of one build (same meaning as the oracle's). */
 typedef struct pg_stats {
   uint64_t prune_dist;    /* node->candidate distances in prune steps */
-  uint64_t prune_pairs;   /* occlusion pair distances in prune steps */
+  uint64_t prune_pairs;   /* occlusion pair distances in prune steps (the
+                             screened link-row sweep counts its screened
+                             pair decisions instead) */
   uint64_t merge_dist;    /* node->candidate distances in merge steps */
   uint64_t merge_pairs;   /* occlusion pair distances in merge steps */
   uint64_t search_dist;   /* visited-miss distance evaluations */

@@ -135,10 +135,14 @@ uint32_t effective_table_size(const pg_params& p) {
 //   bit 5     persistent search grid fed by a device work counter
 //   bit 6     capture the pipeline in a hipGraph once, replay it per build
 //   bit 7     split-BF16 screen in prune / merge (x = hi + lo, tighter
-//             bound, twice the screen bytes)
+//             bound, twice the screen bytes); the link-row prune then runs
+//             its sweep on exact FP32 chains
+//   bit 8     link-row prune candidate by candidate (the reference's loop
+//             order) instead of the kept-major sweep
 constexpr uint32_t kTunePersistent = 0x20u;
 constexpr uint32_t kTuneGraph = 0x40u;
 constexpr uint32_t kTuneSplit = 0x80u;
+constexpr uint32_t kTuneScanPrune = 0x100u;
 uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
 
 // Bound factor of the BF16 screen against the exact FP32 chain:
@@ -232,7 +236,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_prune(d_base, N, p.dim, nullptr, p.gt_k, d_gt,
                              c->winner, p.gt_k, p.proj_pr, c->pruned, c->pcnt,
-                             c->stats, nullptr, s));
+                             c->stats, nullptr, false, s));
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.proj_pr, c->deg,
                                c->off, c->cursor, c->scan_tmp, c->srcs, s));
@@ -262,7 +266,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, width, nullptr,
                                nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
-                               scp, s));
+                               scp, !(p.tune_flags & kTuneScanPrune), s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.M, c->deg, c->off,
                                  c->cursor, c->scan_tmp, c->srcs, s));
@@ -674,7 +678,8 @@ int pg_run_prune(pg_ctx* c, const float* d_base, const uint32_t* d_rows,
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
   PG_HIP(c, pg::launch_prune(d_base, c->p.n_base, c->p.dim, d_rows, D, d_gt,
                              d_winner, gt_k, mp, d_pruned, d_pcnt, c->stats,
-                             c->base16 ? &sc : nullptr, c->stream));
+                             c->base16 ? &sc : nullptr,
+                             !(c->p.tune_flags & kTuneScanPrune), c->stream));
   return PG_OK;
 }
 

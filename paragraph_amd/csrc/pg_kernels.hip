@@ -450,6 +450,435 @@ __global__ __launch_bounds__(64) void prune_kernel(
   }
 }
 
+// ------------------------------------------ K2 on link rows: kept-major sweep
+// The strict RNG keeps a candidate when no earlier kept entry occludes it.
+// Taken kept-major, that reads: the remaining entry with the smallest key is
+// always kept (every entry that an earlier kept entry occludes has already
+// been removed), and it removes every remaining entry that it occludes.
+// That gives prune_kernel's kept lists exactly (same entries in the same
+// order, the same cap, the same stop at the first tomb / INT_MAX /
+// out-of-range id in key order, self entries at FLT_MAX by position), while
+// each sweep reads every remaining candidate row once.  On inner-product
+// rows the first kept entry occludes almost every candidate, so one sweep
+// replaces prune_kernel's distance pass and RNG pass.
+//
+// kScreen (BF16 rows, residual-norm bound): one MFMA per chunk and 16 rows
+// gives the screened product of each row with the current kept entry (B
+// column 0) and, on the first sweep, with the node itself (column 1: the
+// screened d).  pair_bound makes both intervals; an open decision falls back
+// to exact FP32 chains, and only entries that may hold the smallest key get
+// an exact d.  The first sweep is made against the row's first candidate
+// (search rows are sorted, so that is nearly always the smallest key) and is
+// repeated against the right entry when it was not.
+// !kScreen: fused lane-per-row chains give the exact d and the exact pair
+// distance in one pass over each row; the pair counter then equals
+// prune_kernel's candidate-major count.
+
+// chain_dist(x, p) and chain_dist(y, p) in one pass over p: two independent
+// FMA chains in index order, bit-equal to the two separate calls.
+__device__ __forceinline__ void chain_dist2(const float* __restrict__ x,
+                                            const float* __restrict__ y,
+                                            const float* __restrict__ p,
+                                            uint32_t dim, float& dx,
+                                            float& dy) {
+  float s = 0.0f, t = 0.0f;
+  if ((dim & 3u) == 0) {
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const float4* y4 = reinterpret_cast<const float4*>(y);
+    const float4* p4 = reinterpret_cast<const float4*>(p);
+    const uint32_t n4 = dim >> 2;
+    uint32_t i = 0;
+    for (; i + 4 <= n4; i += 4) {
+      float4 a[4], c[4], v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        v[u] = p4[i + u];
+        a[u] = x4[i + u];
+        c[u] = y4[i + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s = __builtin_fmaf(a[u].x, v[u].x, s);
+        t = __builtin_fmaf(c[u].x, v[u].x, t);
+        s = __builtin_fmaf(a[u].y, v[u].y, s);
+        t = __builtin_fmaf(c[u].y, v[u].y, t);
+        s = __builtin_fmaf(a[u].z, v[u].z, s);
+        t = __builtin_fmaf(c[u].z, v[u].z, t);
+        s = __builtin_fmaf(a[u].w, v[u].w, s);
+        t = __builtin_fmaf(c[u].w, v[u].w, t);
+      }
+    }
+    for (; i < n4; ++i) {
+      const float4 a = x4[i], c = y4[i], v = p4[i];
+      s = __builtin_fmaf(a.x, v.x, s);
+      t = __builtin_fmaf(c.x, v.x, t);
+      s = __builtin_fmaf(a.y, v.y, s);
+      t = __builtin_fmaf(c.y, v.y, t);
+      s = __builtin_fmaf(a.z, v.z, s);
+      t = __builtin_fmaf(c.z, v.z, t);
+      s = __builtin_fmaf(a.w, v.w, s);
+      t = __builtin_fmaf(c.w, v.w, t);
+    }
+  } else {
+    for (uint32_t i = 0; i < dim; ++i) {
+      const float v = p[i];
+      s = __builtin_fmaf(x[i], v, s);
+      t = __builtin_fmaf(y[i], v, t);
+    }
+  }
+  dx = -s;
+  dy = -t;
+}
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = shfl_xor_u64(v, uint32_t(o));
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ float wave_min_f32(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <uint32_t DP2, int NCH, bool kScreen>
+__global__ __launch_bounds__(64) void prune_sweep_kernel(
+    const float* __restrict__ base, uint32_t n_base, uint32_t dim,
+    const uint32_t* __restrict__ rows, uint32_t D, uint32_t mp,
+    uint32_t* __restrict__ pruned, uint32_t* __restrict__ pcnt,
+    unsigned long long* __restrict__ stats, ScreenArgs sc) {
+  constexpr int G = NCH <= 8 ? 2 : 1;            // 16-row groups in flight
+  constexpr uint32_t NF = kScreen ? NCH * 4 : 1;  // fragments per row
+  // Entries e (in row order): valid candidates, and self entries before the
+  // first stop id.  Key of e = (ordered d, position << 9 | e).
+  __shared__ uint32_t sid[DP2];
+  __shared__ uint32_t eid[DP2];
+  __shared__ uint16_t epos[DP2];
+  __shared__ uint16_t alive[DP2];
+  __shared__ float dv[DP2];   // exact d when (ef & 1), else the screened d
+  __shared__ float sv[DP2];   // pair value against the current kept entry
+  __shared__ float eb[kScreen ? DP2 : 1];     // bound of the screened d
+  __shared__ float2 enr[kScreen ? DP2 : 1];   // (norm, residual norm)
+  __shared__ uint8_t ef[DP2];  // bit 0: d exact; bit 1: kept or removed
+  __shared__ uint16_t rk[DP2]; // remover's kept index; 0x8000 | t if kept t
+  __shared__ uint32_t kept[128];
+  __shared__ bf16x8 frag[2 * NF];  // [0, NF): kept entry; [NF, 2NF): node
+  const uint32_t b = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t m = lane & 15u, kq = lane >> 4;
+  const float* xb = base + size_t(b) * dim;
+  const uint32_t nch = kScreen ? sc.dpad / 32u : 0u;
+
+  uint32_t jstop = D;
+  for (uint32_t j0 = 0; j0 < D; j0 += kWave) {
+    const uint32_t j = j0 + lane;
+    const uint32_t id = j < D ? rows[size_t(b) * D + j] : kTomb;
+    if (j < D) sid[j] = id;
+    const uint64_t sm = __ballot(j < D && is_stop_id(id, n_base));
+    if (sm && jstop == D) jstop = j0 + __ffsll((long long)sm) - 1;
+  }
+  __syncthreads();
+  uint32_t ne = 0, nv = 0;
+  for (uint32_t j0 = 0; j0 < D; j0 += kWave) {
+    const uint32_t j = j0 + lane;
+    const uint32_t id = j < D ? sid[j] : kTomb;
+    const bool stop = j >= D || is_stop_id(id, n_base);
+    const bool self = !stop && id == b;
+    const bool valid = !stop && !self;
+    const bool inc = valid || (self && j < jstop);
+    const uint64_t im = __ballot(inc);
+    nv += __popcll(__ballot(valid));
+    if (inc) {
+      const uint32_t e = ne + __popcll(im & lanemask_lt());
+      eid[e] = id;
+      epos[e] = uint16_t(j);
+      alive[e] = uint16_t(e);
+      ef[e] = self ? 1u : 0u;
+      dv[e] = self ? FLT_MAX : 0.0f;
+      rk[e] = 0xFFFFu;
+      if constexpr (kScreen) enr[e] = make_float2(sc.norms[id], sc.rnorms[id]);
+    }
+    ne += __popcll(im);
+  }
+  const uint64_t kstop =
+      jstop < D ? ((uint64_t(ordered_bits(FLT_MAX)) << 32) | (jstop << 9))
+                : kPadKey;
+  float nb = 0.0f, rb = 0.0f;
+  auto load_frag = [&](uint32_t slot, uint32_t id) {
+    if constexpr (kScreen) {
+      if (lane < NF) {
+        bf16x8 f;
+        if (lane < 4u * nch) {
+          f = reinterpret_cast<const bf16x8*>(sc.base16 +
+                                              size_t(id) * sc.dpad)[lane];
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) f[j] = (__bf16)0.0f;
+        }
+        frag[slot + lane] = f;
+      }
+    }
+  };
+  if constexpr (kScreen) {
+    nb = sc.norms[b];
+    rb = sc.rnorms[b];
+    load_frag(NF, b);
+  }
+  __syncthreads();
+
+  auto key_of = [&](uint32_t e) -> uint64_t {
+    return (uint64_t(ordered_bits(dv[e])) << 32) | (uint32_t(epos[e]) << 9) | e;
+  };
+  auto lo_of = [&](uint32_t e) -> float {
+    if (kScreen && !(ef[e] & 1u)) return dv[e] - eb[e];
+    return dv[e];
+  };
+  auto hi_of = [&](uint32_t e) -> float {
+    if (kScreen && !(ef[e] & 1u)) return dv[e] + eb[e];
+    return dv[e];
+  };
+  unsigned long long my_pairs = 0, my_fb = 0;
+  uint32_t nal = ne;
+
+  // Pair values of every alive entry against entry c (frag[0, NF) holds c's
+  // fragments in the screened kernel); on the first sweep also d.
+  auto sweep = [&](uint32_t c, bool first) {
+    if constexpr (kScreen) {
+      for (uint32_t g0 = 0; g0 < nal; g0 += 16u * G) {
+        bf16x8 a[G][NCH];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const uint32_t idx = g0 + 16u * g + m;
+          const bool ok = idx < nal;
+          const uint32_t id = ok ? eid[alive[idx]] : 0u;
+          const bf16x8* row = reinterpret_cast<const bf16x8*>(
+              sc.base16 + size_t(id) * sc.dpad + 8u * kq);
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch) {
+            if (ok && uint32_t(ch) < nch) {
+              a[g][ch] = row[4 * ch];
+            } else {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) a[g][ch][j] = (__bf16)0.0f;
+            }
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+          const bf16x8* bf = frag + (m == 1u ? NF : 0u) + kq;
+          __asm__ volatile("" ::: "memory");
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g][ch], bf[4 * ch],
+                                                          acc, 0, 0, 0);
+          // lane l holds rows 4(l/16) .. +3 of column l%16
+          if (m < 2u) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const uint32_t idx = g0 + 16u * g + 4u * kq + j;
+              if (idx < nal) {
+                const uint32_t e = alive[idx];
+                if (m == 0u)
+                  sv[e] = -acc[j];
+                else if (first && !(ef[e] & 1u))
+                  dv[e] = -acc[j];
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (first)
+        for (uint32_t i = lane; i < nal; i += kWave) {
+          const uint32_t e = alive[i];
+          eb[e] = pair_bound(sc, enr[e].x, enr[e].y, nb, rb);
+        }
+    } else {
+      const float* xc = base + size_t(eid[c]) * dim;
+      for (uint32_t i = lane; i < nal; i += kWave) {
+        const uint32_t e = alive[i];
+        const float* xp = base + size_t(eid[e]) * dim;
+        float s;
+        if (first && !(ef[e] & 1u)) {
+          float d;
+          chain_dist2(xb, xc, xp, dim, d, s);
+          dv[e] = d;
+          ef[e] |= 1u;
+        } else {
+          s = chain_dist(xc, xp, dim);
+        }
+        sv[e] = s;
+      }
+    }
+    __syncthreads();
+  };
+
+  // The alive entry with the smallest exact key.  Only entries whose lower
+  // bound does not exceed the smallest upper bound can hold it; with one
+  // such entry no exact d is needed, otherwise those get exact chains.
+  auto pick = [&]() -> uint32_t {
+    float mh = INFINITY;
+    for (uint32_t i = lane; i < nal; i += kWave) mh = fminf(mh, hi_of(alive[i]));
+    mh = wave_min_f32(mh);
+    uint32_t nt = 0, first_t = 0;
+    for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
+      const uint32_t i = i0 + lane;
+      uint32_t e = 0;
+      bool in = false;
+      if (i < nal) {
+        e = alive[i];
+        in = !(lo_of(e) > mh);
+      }
+      const uint64_t tm = __ballot(in);
+      if (tm && nt == 0) first_t = __shfl(e, __ffsll((long long)tm) - 1, 64);
+      nt += __popcll(tm);
+    }
+    if (nt == 1) return first_t;
+    uint64_t best = kPadKey;
+    for (uint32_t i = lane; i < nal; i += kWave) {
+      const uint32_t e = alive[i];
+      if (lo_of(e) > mh) continue;
+      if (!(ef[e] & 1u)) {
+        dv[e] = chain_dist(xb, base + size_t(eid[e]) * dim, dim);
+        ef[e] |= 1u;
+        ++my_fb;
+      }
+      const uint64_t k = key_of(e);
+      best = k < best ? k : best;
+    }
+    best = wave_min_u64(best);
+    __syncthreads();
+    return uint32_t(best & 511u);
+  };
+
+  uint32_t nk = 0, clast = 0;
+  if (ne > 0) {
+    // speculative first kept entry: the first valid candidate in row order
+    uint32_t c = ne;
+    for (uint32_t i0 = 0; i0 < ne && c == ne; i0 += kWave) {
+      const uint32_t i = i0 + lane;
+      const uint64_t vm = __ballot(i < ne && !(ef[i] & 1u));
+      if (vm) c = i0 + __ffsll((long long)vm) - 1;
+    }
+    if (c == ne) c = 0;
+    load_frag(0, eid[c]);
+    __syncthreads();
+    sweep(c, true);
+    const uint32_t c1 = pick();
+    if (c1 != c) {
+      c = c1;
+      load_frag(0, eid[c]);
+      __syncthreads();
+      sweep(c, false);
+    }
+    while (true) {
+      // stop rule: the first stop id in key order ends the scan
+      if (kstop != kPadKey && !(ef[c] & 1u) && !(hi_of(c) < FLT_MAX)) {
+        if (lane == 0) {
+          dv[c] = chain_dist(xb, base + size_t(eid[c]) * dim, dim);
+          ef[c] |= 1u;
+          ++my_fb;
+        }
+        __syncthreads();
+      }
+      if ((ef[c] & 1u) && !(key_of(c) < kstop)) break;
+      if (lane == 0) {
+        kept[nk] = eid[c];
+        rk[c] = uint16_t(0x8000u | nk);
+        ef[c] |= 2u;
+      }
+      ++nk;
+      clast = c;
+      __syncthreads();
+      if (nk == mp) break;
+      // every alive entry against the new kept entry c
+      const float* xc = base + size_t(eid[c]) * dim;
+      float nc = 0.0f, rc = 0.0f;
+      if constexpr (kScreen) {
+        nc = enr[c].x;
+        rc = enr[c].y;
+      }
+      for (uint32_t i = lane; i < nal; i += kWave) {
+        const uint32_t e = alive[i];
+        if (e == c) continue;
+        bool occl;
+        if constexpr (kScreen) {
+          ++my_pairs;
+          const float s = sv[e];
+          const float ec = pair_bound(sc, enr[e].x, enr[e].y, nc, rc);
+          if (s + ec < lo_of(e)) {
+            occl = true;
+          } else if (s - ec >= hi_of(e)) {
+            occl = false;
+          } else {
+            const float* xe = base + size_t(eid[e]) * dim;
+            if (!(ef[e] & 1u)) {
+              dv[e] = chain_dist(xb, xe, dim);
+              ef[e] |= 1u;
+              ++my_fb;
+            }
+            int dec = screen_cmp(s, dv[e], ec);
+            if (dec < 0) {
+              ++my_fb;
+              dec = chain_dist(xc, xe, dim) < dv[e];
+            }
+            occl = dec != 0;
+          }
+        } else {
+          occl = sv[e] < dv[e];
+        }
+        if (occl) {
+          ef[e] |= 2u;
+          rk[e] = uint16_t(nk - 1);
+        }
+      }
+      __syncthreads();
+      uint32_t n2 = 0;
+      for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        const uint32_t e = i < nal ? alive[i] : 0u;
+        const bool keep = i < nal && !(ef[e] & 2u);
+        const uint64_t km = __ballot(keep);
+        if (keep) alive[n2 + __popcll(km & lanemask_lt())] = uint16_t(e);
+        n2 += __popcll(km);
+      }
+      nal = n2;
+      __syncthreads();
+      if (nal == 0) break;
+      c = pick();
+      load_frag(0, eid[c]);
+      __syncthreads();
+      sweep(c, false);
+    }
+  }
+  if constexpr (!kScreen) {
+    // candidate-major pair count: entries up to the last processed key
+    const uint64_t kend = nk == mp ? key_of(clast) : kstop;
+    for (uint32_t e = lane; e < ne; e += kWave) {
+      const uint32_t r = rk[e];
+      if (r == 0xFFFFu) continue;
+      if (r & 0x8000u)
+        my_pairs += r & 0x7FFFu;
+      else if (key_of(e) < kend)
+        my_pairs += r + 1u;
+    }
+  }
+  for (uint32_t j = lane; j < nk; j += kWave) pruned[size_t(b) * mp + j] = kept[j];
+  const unsigned long long pairs = wave_sum<unsigned long long>(my_pairs);
+  const unsigned long long fbk = wave_sum<unsigned long long>(my_fb);
+  if (lane == 0) {
+    pcnt[b] = nk;
+    stat_add(stats, ST_PRUNE_DIST, nv);
+    stat_add(stats, ST_PRUNE_PAIRS, pairs);
+    stat_add(stats, ST_SCREEN_FALLBACK, fbk);
+  }
+}
+
 // ------------------------------------------------------ reverse edges (CSR)
 // add_reverse (Gbuilder.cuh:1020-1038) keeps whichever sources win the
 // atomicAdd race; here every edge b -> c is recorded in a CSR keyed by c and
@@ -1324,10 +1753,33 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
                         const uint32_t* winner, uint32_t gt_k, uint32_t mp,
                         uint32_t* pruned, uint32_t* pcnt,
                         unsigned long long* stats, const ScreenArgs* sc,
-                        hipStream_t s) {
+                        bool sweep, hipStream_t s) {
   if (n_base == 0) return hipSuccess;
   const ScreenArgs scv =
       sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, nullptr, 0, 0.0f, 0.0f, 0.0f};
+  if (D > 512) return hipErrorInvalidValue;
+  if (rows != nullptr && sweep) {
+    // kept-major sweep: screened on the plain BF16 bound; exact FP32 chains
+    // for the FP32 path and for split-screen contexts (whose hi + lo rows
+    // cost as many bytes as the FP32 row)
+#define PG_SWEEP(DP, NC, SCR)                                        \
+  prune_sweep_kernel<DP, NC, SCR><<<n_base, 64, 0, s>>>(              \
+      base, n_base, dim, rows, D, mp, pruned, pcnt, stats, scv)
+    if (sc == nullptr || sc->base16lo != nullptr) {
+      if (D <= 256) PG_SWEEP(256, 0, false);
+      else PG_SWEEP(512, 0, false);
+    } else if (sc->dpad <= 256) {
+      if (D <= 256) PG_SWEEP(256, 8, true);
+      else PG_SWEEP(512, 8, true);
+    } else if (sc->dpad <= 512) {
+      if (D <= 256) PG_SWEEP(256, 16, true);
+      else PG_SWEEP(512, 16, true);
+    } else {
+      return hipErrorInvalidValue;
+    }
+#undef PG_SWEEP
+    return hipGetLastError();
+  }
 #define PG_PRUNE(DP, NC, NK, SP)                                           \
   prune_kernel<DP, NC, NK, SP><<<n_base, 64, 0, s>>>(                      \
       base, n_base, dim, rows, D, gt, winner, gt_k, mp, pruned, pcnt, stats, \
