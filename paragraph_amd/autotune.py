@@ -5,6 +5,7 @@ Knobs (pg_params.tune_flags and table_size):
     screened search, 16-candidate MFMA groups in flight (1 or 2);
   - persistent search grid on / off;
   - split-BF16 screen in the screened prune / merge on / off;
+  - link-row prune: kept-major sweep or candidate-major scan;
   - visited-table size (FP32: exact set, any size above the bound gives the
     same rows; screened: visited-cache size, 2x / 4x / 8x (Km + Kp*Kd)).
 Beam knobs change results: tune_beam picks the beam width of a screened
@@ -14,7 +15,7 @@ keeps the reference mapping.
 
 Method: one default build gives a realistic graph; then the search stage is
 timed on a sample of base rows for every search knob set, and the prune
-stage on the resulting rows for the screen variants.  The fastest set wins.
+stage on the resulting rows for the prune variants.  The fastest set wins.
 Results are cached in a JSON file keyed by shape, precision and device.
 """
 import json
@@ -101,29 +102,34 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             log(f"autotune search {kn}: {1000 * t:.2f} ms")
         if t < best_t:
             best, best_t = dict(kn), t
-    if p.precision != _native.PG_FP32:
-        rows = torch.empty((m, p.topk), **i32)
-        with _native.Context(p) as ctx:
-            ctx.run_search(q, base_d, link, rows)
-            ctx.sync()
-        full = torch.full((n, p.topk), -1, **i32)
-        full[:m] = rows
-        pr = torch.empty((n, p.M), **i32)
-        pc = torch.empty(n, **i32)
-        best_split, best_pt = 0, float("inf")
-        for split in (0, 0x80):
+    # prune variants on real search rows of the sample (other rows empty):
+    # kept-major sweep or candidate-major scan, and for screened builds the
+    # split screen
+    rows = torch.empty((m, p.topk), **i32)
+    with _native.Context(p) as ctx:
+        ctx.run_search(q, base_d, link, rows)
+        ctx.sync()
+    full = torch.full((n, p.topk), -1, **i32)
+    full[:m] = rows
+    pr = torch.empty((n, p.M), **i32)
+    pc = torch.empty(n, **i32)
+    splits = (0, 0x80) if p.precision != _native.PG_FP32 else (0,)
+    best_pf, best_pt = 0, float("inf")
+    for split in splits:
+        for scan in (0, 0x100):
             pp = _native.PgParams.from_buffer_copy(p)
-            pp.tune_flags = (p.tune_flags & ~0x80) | split
+            pp.tune_flags = (p.tune_flags & ~0x180) | split | scan
             with _native.Context(pp) as ctx:
                 t = _time(lambda: (ctx.run_prune_rows(base_d, full, p.M, pr,
                                                       pc), ctx.sync()))
-            report["prune"].append({"split": bool(split),
+            report["prune"].append({"split": bool(split), "scan": bool(scan),
                                     "ms": round(1000 * t, 3)})
             if log:
-                log(f"autotune prune split={bool(split)}: {1000 * t:.2f} ms")
+                log(f"autotune prune split={bool(split)} scan={bool(scan)}: "
+                    f"{1000 * t:.2f} ms")
             if t < best_pt:
-                best_split, best_pt = split, t
-        best["tune_flags"] |= best_split
+                best_pf, best_pt = split | scan, t
+    best["tune_flags"] |= best_pf
     report["best"] = best
     report["seconds"] = round(time.perf_counter() - t_start, 2)
     if use_cache:
@@ -139,7 +145,7 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
 def apply(p, best):
     """Copy of p with the tuned knobs."""
     pp = _native.PgParams.from_buffer_copy(p)
-    pp.tune_flags = (p.tune_flags & ~0xBF) | best["tune_flags"]
+    pp.tune_flags = (p.tune_flags & ~0x1BF) | best["tune_flags"]
     pp.table_size = best["table_size"]
     return pp
 

@@ -174,6 +174,21 @@ __device__ __forceinline__ float pair_bound(const ScreenArgs& sc, float nx,
                   sc.gam_c * nx * ny);
 }
 
+// Bound on |x^.y^ + x^.bf16(y - y^) - chain(x, y)|: the row x in BF16
+// against a column y carried with its BF16 residual (two MFMA columns, summed
+// in FP32).  Error terms: r_x.y <= r_x n_y; x^.(r_y - bf16(r_y)) <= h_x (2^-8
+// r_y + sqrt(dpad) 2^-126) (BF16 rounding of the residual, flushed
+// subnormals included); the FP32 accumulation of both chains and their sum,
+// (gam_a + 2^-24) h_x (h_y + 1.01 r_y); the exact chain's own gam_c n_x n_y;
+// 1% margin for the FP32 norms and this evaluation.
+__device__ __forceinline__ float pair_bound_res(const ScreenArgs& sc, float nx,
+                                                float rx, float ny, float ry) {
+  const float hx = nx + rx, hy = ny + ry;
+  return 1.01f * (rx * ny + hx * (0.00390625f * ry + 1e-36f) +
+                  (sc.gam_a + 5.9604645e-8f) * hx * (hy + 1.01f * ry) +
+                  sc.gam_c * nx * ny);
+}
+
 template <bool kStopIds, int NCH, int NKT, bool kSplit>
 __device__ uint32_t rng_select_screened(
     const float* __restrict__ base, const ScreenArgs& sc, uint32_t n_base,
@@ -450,6 +465,25 @@ __global__ __launch_bounds__(64) void prune_kernel(
   }
 }
 
+// Elements e0 .. e0+7 of a query row (zero past dim or when !on): two
+// 16-byte loads when the row start is 16-byte aligned and dim % 8 == 0,
+// else scalar loads.
+__device__ __forceinline__ void load_query_octet(const float* __restrict__ x,
+                                                 uint32_t dim, bool on,
+                                                 uint32_t e0, float (&v)[8]) {
+  if (on && (dim & 7u) == 0 && e0 + 8u <= dim &&
+      (reinterpret_cast<uintptr_t>(x) & 15u) == 0) {
+    const float4 a = *reinterpret_cast<const float4*>(x + e0);
+    const float4 b = *reinterpret_cast<const float4*>(x + e0 + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      v[j] = (on && e0 + j < dim) ? x[e0 + j] : 0.0f;
+  }
+}
+
 // ------------------------------------------ K2 on link rows: kept-major sweep
 // The strict RNG keeps a candidate when no earlier kept entry occludes it.
 // Taken kept-major, that reads: the remaining entry with the smallest key is
@@ -462,12 +496,12 @@ __global__ __launch_bounds__(64) void prune_kernel(
 // rows the first kept entry occludes almost every candidate, so one sweep
 // replaces prune_kernel's distance pass and RNG pass.
 //
-// kScreen (BF16 rows, residual-norm bound): one MFMA per chunk and 16 rows
-// gives the screened product of each row with the current kept entry (B
-// column 0) and, on the first sweep, with the node itself (column 1: the
-// screened d).  pair_bound makes both intervals; an open decision falls back
-// to exact FP32 chains, and only entries that may hold the smallest key get
-// an exact d.  The first sweep is made against the row's first candidate
+// kScreen (BF16 rows): one MFMA per chunk and 16 rows gives the screened
+// product of each row with the current kept entry (B columns 0 and 2: the
+// entry and its BF16 residual) and, on the first sweep, with the node itself
+// (columns 1 and 3: the screened d).  pair_bound_res makes both intervals; an
+// open decision falls back to exact FP32 chains, and only entries that may
+// hold the smallest key get an exact d.  The first sweep is made against the row's first candidate
 // (search rows are sorted, so that is nearly always the smallest key) and is
 // repeated against the right entry when it was not.
 // !kScreen: fused lane-per-row chains give the exact d and the exact pair
@@ -530,6 +564,35 @@ __device__ __forceinline__ void chain_dist2(const float* __restrict__ x,
   dy = -t;
 }
 
+// chain_dist(x, y) with x in LDS and y in global memory (dim % 4 == 0):
+// the same FMA chain in index order; y is loaded 16 float4 at a time (the
+// tail batch guarded), so a row costs ceil(dim / 64) load round trips.
+__device__ __forceinline__ float chain_dist_l(const float* xl,
+                                              const float* __restrict__ y,
+                                              uint32_t dim) {
+  const float4* x4 = reinterpret_cast<const float4*>(xl);
+  const float4* y4 = reinterpret_cast<const float4*>(y);
+  const uint32_t n4 = dim >> 2;
+  float s = 0.0f;
+  for (uint32_t i = 0; i < n4; i += 16) {
+    float4 b[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (i + u < n4) b[u] = y4[i + u];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      if (i + u < n4) {
+        const float4 a = x4[i + u];
+        s = __builtin_fmaf(a.x, b[u].x, s);
+        s = __builtin_fmaf(a.y, b[u].y, s);
+        s = __builtin_fmaf(a.z, b[u].z, s);
+        s = __builtin_fmaf(a.w, b[u].w, s);
+      }
+    }
+  }
+  return -s;
+}
+
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -545,34 +608,338 @@ __device__ __forceinline__ float wave_min_f32(float v) {
   return v;
 }
 
+// LDS of one sweep (one wave).  Entries e < ne; key of e = (ordered d,
+// position << 9 | e).
+template <uint32_t DP2, int NCH, bool kScreen>
+struct SweepLds {
+  static constexpr uint32_t kRow = kScreen ? NCH * 32 : 1;
+  // screened kernel: FP32 rows (zero-padded) of the current kept entry (0)
+  // and the node (1), for its exact fallbacks
+  float row[2][kRow];
+  // B columns: 0 kept entry, 1 node, 2 / 3 their BF16 residuals
+  bf16x8 frag[kScreen ? 4 * NCH * 4 : 1];
+  float2 enr[kScreen ? DP2 : 1];  // (norm, residual norm)
+  uint32_t eid[DP2];
+  float dv[DP2];   // exact d when (ef & 1), else the screened d
+  float sv[DP2];   // pair value against the current kept entry
+  float eb[kScreen ? DP2 : 1];    // bound of the screened d
+  uint32_t kept[128];
+  uint16_t epos[DP2];
+  uint16_t alive[DP2];
+  uint16_t rk[DP2];  // remover's kept index; 0x8000 | t for kept entry t
+  uint8_t ef[DP2];   // bit 0: d exact; bit 1: kept or removed
+};
+
+// Kept-major strict RNG over the entries set up in L (eid, epos, alive = 0 ..
+// ne-1, ef = 1 and dv = exact d for entries whose d is known, rk = 0xFFFF,
+// enr for the screened kernel), ne > 0.  x_node is the node's row, c0 the
+// speculative first kept entry.  Entries whose key is not below kstop end
+// the scan.  Returns the kept count (at most cap), kept ids in L.kept.
+// pairs: exact kernel, the candidate-major comparison count; screened
+// kernel, the screened pair decisions.  fb: exact chains taken.
+template <uint32_t DP2, int NCH, bool kScreen>
+__device__ __forceinline__ uint32_t sweep_select(
+    SweepLds<DP2, NCH, kScreen>& L, const float* __restrict__ base,
+    uint32_t dim, uint32_t node, const ScreenArgs& sc, uint32_t ne,
+    uint32_t cap, uint64_t kstop, uint32_t c, unsigned long long& pairs_out,
+    unsigned long long& fb_out) {
+  constexpr int G = NCH <= 8 ? 2 : 1;            // 16-row groups in flight
+  constexpr uint32_t NF = kScreen ? NCH * 4 : 1;  // fragments per row
+  const uint32_t lane = lane_id();
+  const uint32_t m = lane & 15u, kq = lane >> 4;
+  const float* xb = base + size_t(node) * dim;
+  const uint32_t nch = kScreen ? sc.dpad / 32u : 0u;
+  float nb = 0.0f, rb = 0.0f;
+  // the screened kernel's exact fallbacks read the node's and the kept
+  // entry's rows from LDS
+  const bool lrows = kScreen && (dim & 3u) == 0 && dim <= L.kRow;
+  // screened kernel, column col (0 kept entry / 1 node): its FP32 row into
+  // L.row[col], the BF16 column and its residual column (col + 2): hi =
+  // bf16(v) (the conversion that made the BF16 copy), lo = bf16(v - hi)
+  auto load_frag = [&](uint32_t col, uint32_t id) {
+    const float* x = base + size_t(id) * dim;
+    if constexpr (kScreen) {
+      if (lane < NF) {
+        float v[8];
+        load_query_octet(x, dim, lane < 4u * nch, 8u * lane, v);
+        bf16x8 h, l;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          h[j] = (__bf16)v[j];
+          l[j] = (__bf16)(v[j] - (float)h[j]);
+          L.row[col][8u * lane + j] = v[j];
+        }
+        L.frag[col * NF + lane] = h;
+        L.frag[(col + 2u) * NF + lane] = l;
+      }
+    }
+  };
+  // exact d of entry e, and exact pair distance of e against the kept entry
+  auto exact_d = [&](uint32_t e) -> float {
+    const float* y = base + size_t(L.eid[e]) * dim;
+    return lrows ? chain_dist_l(L.row[1], y, dim) : chain_dist(xb, y, dim);
+  };
+  auto exact_s = [&](const float* xc, uint32_t e) -> float {
+    const float* y = base + size_t(L.eid[e]) * dim;
+    return lrows ? chain_dist_l(L.row[0], y, dim) : chain_dist(xc, y, dim);
+  };
+  if constexpr (kScreen) {
+    nb = sc.norms[node];
+    rb = sc.rnorms[node];
+  }
+  load_frag(1, node);
+  auto key_of = [&](uint32_t e) -> uint64_t {
+    return (uint64_t(ordered_bits(L.dv[e])) << 32) |
+           (uint32_t(L.epos[e]) << 9) | e;
+  };
+  auto lo_of = [&](uint32_t e) -> float {
+    if (kScreen && !(L.ef[e] & 1u)) return L.dv[e] - L.eb[e];
+    return L.dv[e];
+  };
+  auto hi_of = [&](uint32_t e) -> float {
+    if (kScreen && !(L.ef[e] & 1u)) return L.dv[e] + L.eb[e];
+    return L.dv[e];
+  };
+  unsigned long long my_pairs = 0, my_fb = 0;
+  uint32_t nal = ne;
+
+  // Pair values of every alive entry against entry c (the screened kernel
+  // reads c's fragments from L.frag); on the first sweep also d.
+  auto sweep = [&](uint32_t c, bool first) {
+    if constexpr (kScreen) {
+      for (uint32_t g0 = 0; g0 < nal; g0 += 16u * G) {
+        bf16x8 a[G][NCH];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const uint32_t idx = g0 + 16u * g + m;
+          const bool ok = idx < nal;
+          const uint32_t id = ok ? L.eid[L.alive[idx]] : 0u;
+          const bf16x8* row = reinterpret_cast<const bf16x8*>(
+              sc.base16 + size_t(id) * sc.dpad + 8u * kq);
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch) {
+            if (ok && uint32_t(ch) < nch) {
+              a[g][ch] = row[4 * ch];
+            } else {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) a[g][ch][j] = (__bf16)0.0f;
+            }
+          }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+          const bf16x8* bf = L.frag + (m < 4u ? m : 0u) * NF + kq;
+          __asm__ volatile("" ::: "memory");
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g][ch], bf[4 * ch],
+                                                          acc, 0, 0, 0);
+          // lane l holds rows 4(l/16) .. +3 of column l%16; columns 0 / 1
+          // take their residual columns 2 / 3 from lane l + 2
+          float res[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) res[j] = __shfl(acc[j], int(lane) + 2, 64);
+          if (m < 2u) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const uint32_t idx = g0 + 16u * g + 4u * kq + j;
+              if (idx < nal) {
+                const uint32_t e = L.alive[idx];
+                const float v = -(acc[j] + res[j]);
+                if (m == 0u)
+                  L.sv[e] = v;
+                else if (first && !(L.ef[e] & 1u))
+                  L.dv[e] = v;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (first)
+        for (uint32_t i = lane; i < nal; i += kWave) {
+          const uint32_t e = L.alive[i];
+          L.eb[e] = pair_bound_res(sc, L.enr[e].x, L.enr[e].y, nb, rb);
+        }
+    } else {
+      const float* xc = base + size_t(L.eid[c]) * dim;
+      for (uint32_t i = lane; i < nal; i += kWave) {
+        const uint32_t e = L.alive[i];
+        const float* xp = base + size_t(L.eid[e]) * dim;
+        float s;
+        if (first && !(L.ef[e] & 1u)) {
+          float d;
+          chain_dist2(xb, xc, xp, dim, d, s);
+          L.dv[e] = d;
+          L.ef[e] |= 1u;
+        } else {
+          s = chain_dist(xc, xp, dim);
+        }
+        L.sv[e] = s;
+      }
+    }
+    __syncthreads();
+  };
+
+  // The alive entry with the smallest exact key.  Only entries whose lower
+  // bound does not exceed the smallest upper bound can hold it; with one
+  // such entry no exact d is needed, otherwise those get exact chains.
+  auto pick = [&]() -> uint32_t {
+    float mh = INFINITY;
+    for (uint32_t i = lane; i < nal; i += kWave)
+      mh = fminf(mh, hi_of(L.alive[i]));
+    mh = wave_min_f32(mh);
+    uint32_t nt = 0, first_t = 0;
+    for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
+      const uint32_t i = i0 + lane;
+      uint32_t e = 0;
+      bool in = false;
+      if (i < nal) {
+        e = L.alive[i];
+        in = !(lo_of(e) > mh);
+      }
+      const uint64_t tm = __ballot(in);
+      if (tm && nt == 0) first_t = __shfl(e, __ffsll((long long)tm) - 1, 64);
+      nt += __popcll(tm);
+    }
+    if (nt == 1) return first_t;
+    uint64_t best = kPadKey;
+    for (uint32_t i = lane; i < nal; i += kWave) {
+      const uint32_t e = L.alive[i];
+      if (lo_of(e) > mh) continue;
+      if (!(L.ef[e] & 1u)) {
+        L.dv[e] = exact_d(e);
+        L.ef[e] |= 1u;
+        ++my_fb;
+      }
+      const uint64_t k = key_of(e);
+      best = k < best ? k : best;
+    }
+    best = wave_min_u64(best);
+    __syncthreads();
+    return uint32_t(best & 511u);
+  };
+
+  uint32_t nk = 0, clast = 0;
+  load_frag(0, L.eid[c]);
+  __syncthreads();
+  sweep(c, true);
+  const uint32_t c1 = pick();
+  if (c1 != c) {
+    c = c1;
+    load_frag(0, L.eid[c]);
+    __syncthreads();
+    sweep(c, false);
+  }
+  while (true) {
+    // stop rule: an entry whose key is not below kstop ends the scan
+    if (kstop != kPadKey && !(L.ef[c] & 1u) && !(hi_of(c) < FLT_MAX)) {
+      if (lane == 0) {
+        L.dv[c] = exact_d(c);
+        L.ef[c] |= 1u;
+        ++my_fb;
+      }
+      __syncthreads();
+    }
+    if ((L.ef[c] & 1u) && !(key_of(c) < kstop)) break;
+    if (lane == 0) {
+      L.kept[nk] = L.eid[c];
+      L.rk[c] = uint16_t(0x8000u | nk);
+      L.ef[c] |= 2u;
+    }
+    ++nk;
+    clast = c;
+    __syncthreads();
+    if (nk == cap) break;
+    // every alive entry against the new kept entry c
+    const float* xc = base + size_t(L.eid[c]) * dim;
+    float nc = 0.0f, rc = 0.0f;
+    if constexpr (kScreen) {
+      nc = L.enr[c].x;
+      rc = L.enr[c].y;
+    }
+    for (uint32_t i = lane; i < nal; i += kWave) {
+      const uint32_t e = L.alive[i];
+      if (e == c) continue;
+      bool occl;
+      if constexpr (kScreen) {
+        ++my_pairs;
+        const float s = L.sv[e];
+        const float ec = pair_bound_res(sc, L.enr[e].x, L.enr[e].y, nc, rc);
+        if (s + ec < lo_of(e)) {
+          occl = true;
+        } else if (s - ec >= hi_of(e)) {
+          occl = false;
+        } else {
+          if (!(L.ef[e] & 1u)) {
+            L.dv[e] = exact_d(e);
+            L.ef[e] |= 1u;
+            ++my_fb;
+          }
+          int dec = screen_cmp(s, L.dv[e], ec);
+          if (dec < 0) {
+            ++my_fb;
+            dec = exact_s(xc, e) < L.dv[e];
+          }
+          occl = dec != 0;
+        }
+      } else {
+        occl = L.sv[e] < L.dv[e];
+      }
+      if (occl) {
+        L.ef[e] |= 2u;
+        L.rk[e] = uint16_t(nk - 1);
+      }
+    }
+    __syncthreads();
+    uint32_t n2 = 0;
+    for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
+      const uint32_t i = i0 + lane;
+      const uint32_t e = i < nal ? L.alive[i] : 0u;
+      const bool keep = i < nal && !(L.ef[e] & 2u);
+      const uint64_t km = __ballot(keep);
+      if (keep) L.alive[n2 + __popcll(km & lanemask_lt())] = uint16_t(e);
+      n2 += __popcll(km);
+    }
+    nal = n2;
+    __syncthreads();
+    if (nal == 0) break;
+    c = pick();
+    load_frag(0, L.eid[c]);
+    __syncthreads();
+    sweep(c, false);
+  }
+  if constexpr (!kScreen) {
+    // candidate-major pair count: entries up to the last processed key
+    const uint64_t kend = nk == cap ? key_of(clast) : kstop;
+    for (uint32_t e = lane; e < ne; e += kWave) {
+      const uint32_t r = L.rk[e];
+      if (r == 0xFFFFu) continue;
+      if (r & 0x8000u)
+        my_pairs += r & 0x7FFFu;
+      else if (key_of(e) < kend)
+        my_pairs += r + 1u;
+    }
+  }
+  pairs_out = wave_sum<unsigned long long>(my_pairs);
+  fb_out = wave_sum<unsigned long long>(my_fb);
+  return nk;
+}
+
 template <uint32_t DP2, int NCH, bool kScreen>
 __global__ __launch_bounds__(64) void prune_sweep_kernel(
     const float* __restrict__ base, uint32_t n_base, uint32_t dim,
     const uint32_t* __restrict__ rows, uint32_t D, uint32_t mp,
     uint32_t* __restrict__ pruned, uint32_t* __restrict__ pcnt,
     unsigned long long* __restrict__ stats, ScreenArgs sc) {
-  constexpr int G = NCH <= 8 ? 2 : 1;            // 16-row groups in flight
-  constexpr uint32_t NF = kScreen ? NCH * 4 : 1;  // fragments per row
-  // Entries e (in row order): valid candidates, and self entries before the
-  // first stop id.  Key of e = (ordered d, position << 9 | e).
-  __shared__ uint32_t sid[DP2];
-  __shared__ uint32_t eid[DP2];
-  __shared__ uint16_t epos[DP2];
-  __shared__ uint16_t alive[DP2];
-  __shared__ float dv[DP2];   // exact d when (ef & 1), else the screened d
-  __shared__ float sv[DP2];   // pair value against the current kept entry
-  __shared__ float eb[kScreen ? DP2 : 1];     // bound of the screened d
-  __shared__ float2 enr[kScreen ? DP2 : 1];   // (norm, residual norm)
-  __shared__ uint8_t ef[DP2];  // bit 0: d exact; bit 1: kept or removed
-  __shared__ uint16_t rk[DP2]; // remover's kept index; 0x8000 | t if kept t
-  __shared__ uint32_t kept[128];
-  __shared__ bf16x8 frag[2 * NF];  // [0, NF): kept entry; [NF, 2NF): node
+  __shared__ SweepLds<DP2, NCH, kScreen> L;
+  // row ids, read before the first sweep writes L.sv
+  uint32_t* sid = reinterpret_cast<uint32_t*>(L.sv);
   const uint32_t b = blockIdx.x;
   const uint32_t lane = lane_id();
-  const uint32_t m = lane & 15u, kq = lane >> 4;
-  const float* xb = base + size_t(b) * dim;
-  const uint32_t nch = kScreen ? sc.dpad / 32u : 0u;
-
+  // Entries: valid candidates, and self entries before the first stop id
+  // (prune_kernel sorts them at FLT_MAX by position).
   uint32_t jstop = D;
   for (uint32_t j0 = 0; j0 < D; j0 += kWave) {
     const uint32_t j = j0 + lane;
@@ -594,283 +961,36 @@ __global__ __launch_bounds__(64) void prune_sweep_kernel(
     nv += __popcll(__ballot(valid));
     if (inc) {
       const uint32_t e = ne + __popcll(im & lanemask_lt());
-      eid[e] = id;
-      epos[e] = uint16_t(j);
-      alive[e] = uint16_t(e);
-      ef[e] = self ? 1u : 0u;
-      dv[e] = self ? FLT_MAX : 0.0f;
-      rk[e] = 0xFFFFu;
-      if constexpr (kScreen) enr[e] = make_float2(sc.norms[id], sc.rnorms[id]);
+      L.eid[e] = id;
+      L.epos[e] = uint16_t(j);
+      L.alive[e] = uint16_t(e);
+      L.ef[e] = self ? 1u : 0u;
+      L.dv[e] = self ? FLT_MAX : 0.0f;
+      L.rk[e] = 0xFFFFu;
+      if constexpr (kScreen)
+        L.enr[e] = make_float2(sc.norms[id], sc.rnorms[id]);
     }
     ne += __popcll(im);
   }
+  __syncthreads();
   const uint64_t kstop =
       jstop < D ? ((uint64_t(ordered_bits(FLT_MAX)) << 32) | (jstop << 9))
                 : kPadKey;
-  float nb = 0.0f, rb = 0.0f;
-  auto load_frag = [&](uint32_t slot, uint32_t id) {
-    if constexpr (kScreen) {
-      if (lane < NF) {
-        bf16x8 f;
-        if (lane < 4u * nch) {
-          f = reinterpret_cast<const bf16x8*>(sc.base16 +
-                                              size_t(id) * sc.dpad)[lane];
-        } else {
-#pragma unroll
-          for (int j = 0; j < 8; ++j) f[j] = (__bf16)0.0f;
-        }
-        frag[slot + lane] = f;
-      }
-    }
-  };
-  if constexpr (kScreen) {
-    nb = sc.norms[b];
-    rb = sc.rnorms[b];
-    load_frag(NF, b);
-  }
-  __syncthreads();
-
-  auto key_of = [&](uint32_t e) -> uint64_t {
-    return (uint64_t(ordered_bits(dv[e])) << 32) | (uint32_t(epos[e]) << 9) | e;
-  };
-  auto lo_of = [&](uint32_t e) -> float {
-    if (kScreen && !(ef[e] & 1u)) return dv[e] - eb[e];
-    return dv[e];
-  };
-  auto hi_of = [&](uint32_t e) -> float {
-    if (kScreen && !(ef[e] & 1u)) return dv[e] + eb[e];
-    return dv[e];
-  };
-  unsigned long long my_pairs = 0, my_fb = 0;
-  uint32_t nal = ne;
-
-  // Pair values of every alive entry against entry c (frag[0, NF) holds c's
-  // fragments in the screened kernel); on the first sweep also d.
-  auto sweep = [&](uint32_t c, bool first) {
-    if constexpr (kScreen) {
-      for (uint32_t g0 = 0; g0 < nal; g0 += 16u * G) {
-        bf16x8 a[G][NCH];
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          const uint32_t idx = g0 + 16u * g + m;
-          const bool ok = idx < nal;
-          const uint32_t id = ok ? eid[alive[idx]] : 0u;
-          const bf16x8* row = reinterpret_cast<const bf16x8*>(
-              sc.base16 + size_t(id) * sc.dpad + 8u * kq);
-#pragma unroll
-          for (int ch = 0; ch < NCH; ++ch) {
-            if (ok && uint32_t(ch) < nch) {
-              a[g][ch] = row[4 * ch];
-            } else {
-#pragma unroll
-              for (int j = 0; j < 8; ++j) a[g][ch][j] = (__bf16)0.0f;
-            }
-          }
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-          f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-          const bf16x8* bf = frag + (m == 1u ? NF : 0u) + kq;
-          __asm__ volatile("" ::: "memory");
-#pragma unroll
-          for (int ch = 0; ch < NCH; ++ch)
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[g][ch], bf[4 * ch],
-                                                          acc, 0, 0, 0);
-          // lane l holds rows 4(l/16) .. +3 of column l%16
-          if (m < 2u) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const uint32_t idx = g0 + 16u * g + 4u * kq + j;
-              if (idx < nal) {
-                const uint32_t e = alive[idx];
-                if (m == 0u)
-                  sv[e] = -acc[j];
-                else if (first && !(ef[e] & 1u))
-                  dv[e] = -acc[j];
-              }
-            }
-          }
-        }
-      }
-      __syncthreads();
-      if (first)
-        for (uint32_t i = lane; i < nal; i += kWave) {
-          const uint32_t e = alive[i];
-          eb[e] = pair_bound(sc, enr[e].x, enr[e].y, nb, rb);
-        }
-    } else {
-      const float* xc = base + size_t(eid[c]) * dim;
-      for (uint32_t i = lane; i < nal; i += kWave) {
-        const uint32_t e = alive[i];
-        const float* xp = base + size_t(eid[e]) * dim;
-        float s;
-        if (first && !(ef[e] & 1u)) {
-          float d;
-          chain_dist2(xb, xc, xp, dim, d, s);
-          dv[e] = d;
-          ef[e] |= 1u;
-        } else {
-          s = chain_dist(xc, xp, dim);
-        }
-        sv[e] = s;
-      }
-    }
-    __syncthreads();
-  };
-
-  // The alive entry with the smallest exact key.  Only entries whose lower
-  // bound does not exceed the smallest upper bound can hold it; with one
-  // such entry no exact d is needed, otherwise those get exact chains.
-  auto pick = [&]() -> uint32_t {
-    float mh = INFINITY;
-    for (uint32_t i = lane; i < nal; i += kWave) mh = fminf(mh, hi_of(alive[i]));
-    mh = wave_min_f32(mh);
-    uint32_t nt = 0, first_t = 0;
-    for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
-      const uint32_t i = i0 + lane;
-      uint32_t e = 0;
-      bool in = false;
-      if (i < nal) {
-        e = alive[i];
-        in = !(lo_of(e) > mh);
-      }
-      const uint64_t tm = __ballot(in);
-      if (tm && nt == 0) first_t = __shfl(e, __ffsll((long long)tm) - 1, 64);
-      nt += __popcll(tm);
-    }
-    if (nt == 1) return first_t;
-    uint64_t best = kPadKey;
-    for (uint32_t i = lane; i < nal; i += kWave) {
-      const uint32_t e = alive[i];
-      if (lo_of(e) > mh) continue;
-      if (!(ef[e] & 1u)) {
-        dv[e] = chain_dist(xb, base + size_t(eid[e]) * dim, dim);
-        ef[e] |= 1u;
-        ++my_fb;
-      }
-      const uint64_t k = key_of(e);
-      best = k < best ? k : best;
-    }
-    best = wave_min_u64(best);
-    __syncthreads();
-    return uint32_t(best & 511u);
-  };
-
-  uint32_t nk = 0, clast = 0;
+  uint32_t nk = 0;
+  unsigned long long pairs = 0, fbk = 0;
   if (ne > 0) {
     // speculative first kept entry: the first valid candidate in row order
     uint32_t c = ne;
     for (uint32_t i0 = 0; i0 < ne && c == ne; i0 += kWave) {
       const uint32_t i = i0 + lane;
-      const uint64_t vm = __ballot(i < ne && !(ef[i] & 1u));
+      const uint64_t vm = __ballot(i < ne && !(L.ef[i] & 1u));
       if (vm) c = i0 + __ffsll((long long)vm) - 1;
     }
     if (c == ne) c = 0;
-    load_frag(0, eid[c]);
-    __syncthreads();
-    sweep(c, true);
-    const uint32_t c1 = pick();
-    if (c1 != c) {
-      c = c1;
-      load_frag(0, eid[c]);
-      __syncthreads();
-      sweep(c, false);
-    }
-    while (true) {
-      // stop rule: the first stop id in key order ends the scan
-      if (kstop != kPadKey && !(ef[c] & 1u) && !(hi_of(c) < FLT_MAX)) {
-        if (lane == 0) {
-          dv[c] = chain_dist(xb, base + size_t(eid[c]) * dim, dim);
-          ef[c] |= 1u;
-          ++my_fb;
-        }
-        __syncthreads();
-      }
-      if ((ef[c] & 1u) && !(key_of(c) < kstop)) break;
-      if (lane == 0) {
-        kept[nk] = eid[c];
-        rk[c] = uint16_t(0x8000u | nk);
-        ef[c] |= 2u;
-      }
-      ++nk;
-      clast = c;
-      __syncthreads();
-      if (nk == mp) break;
-      // every alive entry against the new kept entry c
-      const float* xc = base + size_t(eid[c]) * dim;
-      float nc = 0.0f, rc = 0.0f;
-      if constexpr (kScreen) {
-        nc = enr[c].x;
-        rc = enr[c].y;
-      }
-      for (uint32_t i = lane; i < nal; i += kWave) {
-        const uint32_t e = alive[i];
-        if (e == c) continue;
-        bool occl;
-        if constexpr (kScreen) {
-          ++my_pairs;
-          const float s = sv[e];
-          const float ec = pair_bound(sc, enr[e].x, enr[e].y, nc, rc);
-          if (s + ec < lo_of(e)) {
-            occl = true;
-          } else if (s - ec >= hi_of(e)) {
-            occl = false;
-          } else {
-            const float* xe = base + size_t(eid[e]) * dim;
-            if (!(ef[e] & 1u)) {
-              dv[e] = chain_dist(xb, xe, dim);
-              ef[e] |= 1u;
-              ++my_fb;
-            }
-            int dec = screen_cmp(s, dv[e], ec);
-            if (dec < 0) {
-              ++my_fb;
-              dec = chain_dist(xc, xe, dim) < dv[e];
-            }
-            occl = dec != 0;
-          }
-        } else {
-          occl = sv[e] < dv[e];
-        }
-        if (occl) {
-          ef[e] |= 2u;
-          rk[e] = uint16_t(nk - 1);
-        }
-      }
-      __syncthreads();
-      uint32_t n2 = 0;
-      for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
-        const uint32_t i = i0 + lane;
-        const uint32_t e = i < nal ? alive[i] : 0u;
-        const bool keep = i < nal && !(ef[e] & 2u);
-        const uint64_t km = __ballot(keep);
-        if (keep) alive[n2 + __popcll(km & lanemask_lt())] = uint16_t(e);
-        n2 += __popcll(km);
-      }
-      nal = n2;
-      __syncthreads();
-      if (nal == 0) break;
-      c = pick();
-      load_frag(0, eid[c]);
-      __syncthreads();
-      sweep(c, false);
-    }
+    nk = sweep_select<DP2, NCH, kScreen>(L, base, dim, b, sc, ne, mp, kstop,
+                                         c, pairs, fbk);
   }
-  if constexpr (!kScreen) {
-    // candidate-major pair count: entries up to the last processed key
-    const uint64_t kend = nk == mp ? key_of(clast) : kstop;
-    for (uint32_t e = lane; e < ne; e += kWave) {
-      const uint32_t r = rk[e];
-      if (r == 0xFFFFu) continue;
-      if (r & 0x8000u)
-        my_pairs += r & 0x7FFFu;
-      else if (key_of(e) < kend)
-        my_pairs += r + 1u;
-    }
-  }
-  for (uint32_t j = lane; j < nk; j += kWave) pruned[size_t(b) * mp + j] = kept[j];
-  const unsigned long long pairs = wave_sum<unsigned long long>(my_pairs);
-  const unsigned long long fbk = wave_sum<unsigned long long>(my_fb);
+  for (uint32_t j = lane; j < nk; j += kWave) pruned[size_t(b) * mp + j] = L.kept[j];
   if (lane == 0) {
     pcnt[b] = nk;
     stat_add(stats, ST_PRUNE_DIST, nv);
@@ -1207,25 +1327,6 @@ struct EvalF32 {
     }
   }
 };
-
-// Elements e0 .. e0+7 of a query row (zero past dim or when !on): two
-// 16-byte loads when the row start is 16-byte aligned and dim % 8 == 0,
-// else scalar loads.
-__device__ __forceinline__ void load_query_octet(const float* __restrict__ x,
-                                                 uint32_t dim, bool on,
-                                                 uint32_t e0, float (&v)[8]) {
-  if (on && (dim & 7u) == 0 && e0 + 8u <= dim &&
-      (reinterpret_cast<uintptr_t>(x) & 15u) == 0) {
-    const float4 a = *reinterpret_cast<const float4*>(x + e0);
-    const float4 b = *reinterpret_cast<const float4*>(x + e0 + 4);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-  } else {
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      v[j] = (on && e0 + j < dim) ? x[e0 + j] : 0.0f;
-  }
-}
 
 // EvalBF16: screening distances on MFMA.  The base is a BF16 copy with rows
 // zero-padded to dpad (a multiple of 32).  16 candidates form the A operand

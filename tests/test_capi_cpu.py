@@ -100,6 +100,10 @@ def test_autotune_candidates_valid():
                 assert getattr(pp, f) == getattr(p, f)
         pp = autotune.apply(p, {"tune_flags": 0xA4, "table_size": 0})
         assert pp.tune_flags == 0xA4
+        # the prune knob (bit 8) is replaced, not or-ed in
+        p.tune_flags = 0x120
+        pp = autotune.apply(p, {"tune_flags": 0x21, "table_size": 0})
+        assert pp.tune_flags == 0x21
 
 
 def test_builder_rejects_unknown_precision():

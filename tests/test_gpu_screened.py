@@ -142,12 +142,16 @@ def _stage_inputs(n, dim, seed=0, integer=False):
     return datagen.base_vectors(n, dim, 64)
 
 
+@pytest.mark.parametrize("scan", [False, True])
 @pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("integer", [False, True])
 @pytest.mark.parametrize("dim,mp", [(200, 32), (512, 64)])
-def test_screened_prune_and_merge_equal_oracle(split, integer, dim, mp, cuda):
+def test_screened_prune_and_merge_equal_oracle(scan, split, integer, dim, mp,
+                                               cuda):
     """BF16 screen + exact fallback must reproduce the FP32 decisions exactly,
-    including on data full of exact ties (fallbacks exercised)."""
+    including on data full of exact ties (fallbacks exercised), with the
+    kept-major prune sweep and the candidate-major prune (scan).  A split
+    screen context runs its prune sweep on exact chains."""
     import torch
     n = 1200
     base = _stage_inputs(n, dim, integer=integer)
@@ -155,7 +159,8 @@ def test_screened_prune_and_merge_equal_oracle(split, integer, dim, mp, cuda):
     rows = rng.integers(0, n, (n, 256)).astype(np.uint32)
     rows[rng.random(rows.shape) < 0.05] = 0x7FFFFFFF
     p = _native.make_params(n, dim, n, mp, 128, precision=_native.PG_BF16,
-                            tune_flags=0x20 | (0x80 if split else 0))
+                            tune_flags=0x20 | (0x80 if split else 0) |
+                            (0x100 if scan else 0))
     base_d = _dev(base, torch)
     pr_d = torch.full((n, mp), -1, dtype=torch.int32, device=cuda)
     pc_d = torch.zeros(n, dtype=torch.int32, device=cuda)
@@ -176,7 +181,7 @@ def test_screened_prune_and_merge_equal_oracle(split, integer, dim, mp, cuda):
         st = ctx.stats()
     out = cpu_ref.merge(base, off, srcs, cap, pr, pc, mp, True)
     np.testing.assert_array_equal(_host(out_d), out)
-    if integer:
+    if integer and (scan or not split):
         assert st.screen_fallback > 0
 
 

@@ -120,3 +120,4 @@ def test_sweep_equals_scan_and_oracle(case, prec, flags, cuda):
     if prec == _native.PG_FP32:
         assert st_s["prune_pairs"] == st_c["prune_pairs"] == st_o.as_dict()[
             "prune_pairs"]
+
