@@ -1003,37 +1003,124 @@ __global__ __launch_bounds__(64) void prune_sweep_kernel(
 // add_reverse (Gbuilder.cuh:1020-1038) keeps whichever sources win the
 // atomicAdd race; here every edge b -> c is recorded in a CSR keyed by c and
 // the merge step keeps the cap smallest sources (rule C2).
-__global__ void reverse_count_kernel(const uint32_t* __restrict__ pruned,
-                                     const uint32_t* __restrict__ pcnt,
-                                     uint32_t n_base, uint32_t mp,
-                                     uint32_t* __restrict__ deg) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n_base;
-       b += gridDim.x * blockDim.x) {
-    const uint32_t cnt = pcnt[b];
-    const uint32_t* row = pruned + size_t(b) * mp;
-    for (uint32_t j = 0; j < cnt; ++j) {
-      const uint32_t c = row[j];
-      if (c < n_base) atomicAdd(&deg[c], 1u);
-    }
-  }
+//
+// Inner-product graphs have hubs (one node can be kept by 2% of all nodes),
+// and device-scope atomics on one address serialize across the XCDs.  So
+// each block takes a contiguous range of kRevNodes nodes, aggregates their
+// edges in an LDS hash table (target -> count) and issues one global atomic
+// per distinct target.  An edge whose target finds no slot within kRevProbe
+// probes goes straight to global memory; because slots are never freed, all
+// edges of one target either share one slot or all overflow, in both passes.
+constexpr uint32_t kRevSlots = 8192;
+constexpr uint32_t kRevNodes = 1024;
+constexpr uint32_t kRevProbe = 32;
+constexpr uint32_t kRevEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t rev_hash(uint32_t c) {
+  return (c * 2654435761u) >> 19;  // 13 bits
 }
 
-__global__ void reverse_fill_kernel(const uint32_t* __restrict__ pruned,
-                                    const uint32_t* __restrict__ pcnt,
-                                    uint32_t n_base, uint32_t mp,
-                                    const uint32_t* __restrict__ off,
-                                    uint32_t* __restrict__ cursor,
-                                    uint32_t* __restrict__ srcs) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < n_base;
-       b += gridDim.x * blockDim.x) {
-    const uint32_t cnt = pcnt[b];
+// Slot of target c (inserted if absent), or kRevSlots when the probe run is
+// full.
+__device__ __forceinline__ uint32_t rev_insert(uint32_t* keys, uint32_t c) {
+  uint32_t h = rev_hash(c);
+  for (uint32_t p = 0; p < kRevProbe; ++p) {
+    const uint32_t prev = atomicCAS(&keys[h], kRevEmpty, c);
+    if (prev == kRevEmpty || prev == c) return h;
+    h = (h + 1u) & (kRevSlots - 1u);
+  }
+  return kRevSlots;
+}
+
+__device__ __forceinline__ uint32_t rev_find(const uint32_t* keys,
+                                             uint32_t c) {
+  uint32_t h = rev_hash(c);
+  for (uint32_t p = 0; p < kRevProbe; ++p) {
+    const uint32_t k = keys[h];
+    if (k == c) return h;
+    if (k == kRevEmpty) return kRevSlots;
+    h = (h + 1u) & (kRevSlots - 1u);
+  }
+  return kRevSlots;
+}
+
+__global__ __launch_bounds__(256) void reverse_count_kernel(
+    const uint32_t* __restrict__ pruned, const uint32_t* __restrict__ pcnt,
+    uint32_t n_base, uint32_t mp, uint32_t* __restrict__ deg) {
+  __shared__ uint32_t keys[kRevSlots];
+  __shared__ uint32_t cnt[kRevSlots];
+  for (uint32_t i = threadIdx.x; i < kRevSlots; i += blockDim.x) {
+    keys[i] = kRevEmpty;
+    cnt[i] = 0u;
+  }
+  __syncthreads();
+  const uint32_t b0 = blockIdx.x * kRevNodes;
+  const uint32_t b1 = min(b0 + kRevNodes, n_base);
+  for (uint32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+    const uint32_t n = pcnt[b];
     const uint32_t* row = pruned + size_t(b) * mp;
-    for (uint32_t j = 0; j < cnt; ++j) {
+    for (uint32_t j = 0; j < n; ++j) {
       const uint32_t c = row[j];
-      if (c < n_base) {
+      if (c >= n_base) continue;
+      const uint32_t h = rev_insert(keys, c);
+      if (h < kRevSlots)
+        atomicAdd(&cnt[h], 1u);
+      else
+        atomicAdd(&deg[c], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kRevSlots; i += blockDim.x)
+    if (keys[i] != kRevEmpty) atomicAdd(&deg[keys[i]], cnt[i]);
+}
+
+// Two passes over the block's edges: the first builds the same table (and
+// places overflow edges directly), then one global atomic per slot reserves
+// the target's run, and the second pass ranks each edge inside its run.
+__global__ __launch_bounds__(256) void reverse_fill_kernel(
+    const uint32_t* __restrict__ pruned, const uint32_t* __restrict__ pcnt,
+    uint32_t n_base, uint32_t mp, const uint32_t* __restrict__ off,
+    uint32_t* __restrict__ cursor, uint32_t* __restrict__ srcs) {
+  __shared__ uint32_t keys[kRevSlots];
+  __shared__ uint32_t cnt[kRevSlots];
+  __shared__ uint32_t run[kRevSlots];
+  for (uint32_t i = threadIdx.x; i < kRevSlots; i += blockDim.x) {
+    keys[i] = kRevEmpty;
+    cnt[i] = 0u;
+  }
+  __syncthreads();
+  const uint32_t b0 = blockIdx.x * kRevNodes;
+  const uint32_t b1 = min(b0 + kRevNodes, n_base);
+  for (uint32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+    const uint32_t n = pcnt[b];
+    const uint32_t* row = pruned + size_t(b) * mp;
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t c = row[j];
+      if (c >= n_base) continue;
+      const uint32_t h = rev_insert(keys, c);
+      if (h < kRevSlots) {
+        atomicAdd(&cnt[h], 1u);
+      } else {
         const uint32_t slot = atomicAdd(&cursor[c], 1u);
         srcs[off[c] + slot] = b;
       }
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kRevSlots; i += blockDim.x) {
+    const uint32_t c = keys[i];
+    if (c != kRevEmpty) run[i] = off[c] + atomicAdd(&cursor[c], cnt[i]);
+    cnt[i] = 0u;
+  }
+  __syncthreads();
+  for (uint32_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+    const uint32_t n = pcnt[b];
+    const uint32_t* row = pruned + size_t(b) * mp;
+    for (uint32_t j = 0; j < n; ++j) {
+      const uint32_t c = row[j];
+      if (c >= n_base) continue;
+      const uint32_t h = rev_find(keys, c);
+      if (h < kRevSlots) srcs[run[h] + atomicAdd(&cnt[h], 1u)] = b;
     }
   }
 }
@@ -1931,14 +2018,14 @@ hipError_t launch_reverse(const uint32_t* pruned, const uint32_t* pcnt,
   if (e != hipSuccess) return e;
   const uint64_t total = uint64_t(n_base) * mp;
   if (total == 0) return launch_fill_u32(off, uint64_t(n_base) + 1, 0u, s);
-  reverse_count_kernel<<<grid_for(n_base, 256), 256, 0, s>>>(pruned, pcnt,
-                                                             n_base, mp, deg);
+  const uint32_t rb = (n_base + kRevNodes - 1) / kRevNodes;
+  reverse_count_kernel<<<rb, 256, 0, s>>>(pruned, pcnt, n_base, mp, deg);
   const uint32_t nb = (n_base + kScanBlock - 1) / kScanBlock;
   scan_blocks_kernel<<<nb, 256, 0, s>>>(deg, n_base, off, scan_tmp);
   scan_sums_kernel<<<1, 1024, 0, s>>>(scan_tmp, nb, off + n_base);
   scan_add_kernel<<<(n_base + 255) / 256, 256, 0, s>>>(off, n_base, scan_tmp);
-  reverse_fill_kernel<<<grid_for(n_base, 256), 256, 0, s>>>(
-      pruned, pcnt, n_base, mp, off, cursor, srcs);
+  reverse_fill_kernel<<<rb, 256, 0, s>>>(pruned, pcnt, n_base, mp, off,
+                                         cursor, srcs);
   return hipGetLastError();
 }
 
