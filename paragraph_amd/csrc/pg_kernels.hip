@@ -564,35 +564,6 @@ __device__ __forceinline__ void chain_dist2(const float* __restrict__ x,
   dy = -t;
 }
 
-// chain_dist(x, y) with x in LDS and y in global memory (dim % 4 == 0):
-// the same FMA chain in index order; y is loaded 16 float4 at a time (the
-// tail batch guarded), so a row costs ceil(dim / 64) load round trips.
-__device__ __forceinline__ float chain_dist_l(const float* xl,
-                                              const float* __restrict__ y,
-                                              uint32_t dim) {
-  const float4* x4 = reinterpret_cast<const float4*>(xl);
-  const float4* y4 = reinterpret_cast<const float4*>(y);
-  const uint32_t n4 = dim >> 2;
-  float s = 0.0f;
-  for (uint32_t i = 0; i < n4; i += 16) {
-    float4 b[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u)
-      if (i + u < n4) b[u] = y4[i + u];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      if (i + u < n4) {
-        const float4 a = x4[i + u];
-        s = __builtin_fmaf(a.x, b[u].x, s);
-        s = __builtin_fmaf(a.y, b[u].y, s);
-        s = __builtin_fmaf(a.z, b[u].z, s);
-        s = __builtin_fmaf(a.w, b[u].w, s);
-      }
-    }
-  }
-  return -s;
-}
-
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -612,17 +583,12 @@ __device__ __forceinline__ float wave_min_f32(float v) {
 // position << 9 | e).
 template <uint32_t DP2, int NCH, bool kScreen>
 struct SweepLds {
-  static constexpr uint32_t kRow = kScreen ? NCH * 32 : 1;
-  // screened kernel: FP32 rows (zero-padded) of the current kept entry (0)
-  // and the node (1), for its exact fallbacks
-  float row[2][kRow];
   // B columns: 0 kept entry, 1 node, 2 / 3 their BF16 residuals
   bf16x8 frag[kScreen ? 4 * NCH * 4 : 1];
   float2 enr[kScreen ? DP2 : 1];  // (norm, residual norm)
   uint32_t eid[DP2];
   float dv[DP2];   // exact d when (ef & 1), else the screened d
   float sv[DP2];   // pair value against the current kept entry
-  float eb[kScreen ? DP2 : 1];    // bound of the screened d
   uint32_t kept[128];
   uint16_t epos[DP2];
   uint16_t alive[DP2];
@@ -650,12 +616,9 @@ __device__ __forceinline__ uint32_t sweep_select(
   const float* xb = base + size_t(node) * dim;
   const uint32_t nch = kScreen ? sc.dpad / 32u : 0u;
   float nb = 0.0f, rb = 0.0f;
-  // the screened kernel's exact fallbacks read the node's and the kept
-  // entry's rows from LDS
-  const bool lrows = kScreen && (dim & 3u) == 0 && dim <= L.kRow;
-  // screened kernel, column col (0 kept entry / 1 node): its FP32 row into
-  // L.row[col], the BF16 column and its residual column (col + 2): hi =
-  // bf16(v) (the conversion that made the BF16 copy), lo = bf16(v - hi)
+  // screened kernel, column col (0 kept entry / 1 node) from the FP32 row:
+  // the BF16 column and its residual column (col + 2): hi = bf16(v) (the
+  // conversion that made the BF16 copy), lo = bf16(v - hi)
   auto load_frag = [&](uint32_t col, uint32_t id) {
     const float* x = base + size_t(id) * dim;
     if constexpr (kScreen) {
@@ -667,7 +630,6 @@ __device__ __forceinline__ uint32_t sweep_select(
         for (int j = 0; j < 8; ++j) {
           h[j] = (__bf16)v[j];
           l[j] = (__bf16)(v[j] - (float)h[j]);
-          L.row[col][8u * lane + j] = v[j];
         }
         L.frag[col * NF + lane] = h;
         L.frag[(col + 2u) * NF + lane] = l;
@@ -676,12 +638,14 @@ __device__ __forceinline__ uint32_t sweep_select(
   };
   // exact d of entry e, and exact pair distance of e against the kept entry
   auto exact_d = [&](uint32_t e) -> float {
-    const float* y = base + size_t(L.eid[e]) * dim;
-    return lrows ? chain_dist_l(L.row[1], y, dim) : chain_dist(xb, y, dim);
+    return chain_dist(xb, base + size_t(L.eid[e]) * dim, dim);
   };
   auto exact_s = [&](const float* xc, uint32_t e) -> float {
-    const float* y = base + size_t(L.eid[e]) * dim;
-    return lrows ? chain_dist_l(L.row[0], y, dim) : chain_dist(xc, y, dim);
+    return chain_dist(xc, base + size_t(L.eid[e]) * dim, dim);
+  };
+  // bound of the screened d of entry e
+  auto eb_of = [&](uint32_t e) -> float {
+    return pair_bound_res(sc, L.enr[e].x, L.enr[e].y, nb, rb);
   };
   if constexpr (kScreen) {
     nb = sc.norms[node];
@@ -693,11 +657,13 @@ __device__ __forceinline__ uint32_t sweep_select(
            (uint32_t(L.epos[e]) << 9) | e;
   };
   auto lo_of = [&](uint32_t e) -> float {
-    if (kScreen && !(L.ef[e] & 1u)) return L.dv[e] - L.eb[e];
+    if constexpr (kScreen)
+      if (!(L.ef[e] & 1u)) return L.dv[e] - eb_of(e);
     return L.dv[e];
   };
   auto hi_of = [&](uint32_t e) -> float {
-    if (kScreen && !(L.ef[e] & 1u)) return L.dv[e] + L.eb[e];
+    if constexpr (kScreen)
+      if (!(L.ef[e] & 1u)) return L.dv[e] + eb_of(e);
     return L.dv[e];
   };
   unsigned long long my_pairs = 0, my_fb = 0;
@@ -756,12 +722,6 @@ __device__ __forceinline__ uint32_t sweep_select(
           }
         }
       }
-      __syncthreads();
-      if (first)
-        for (uint32_t i = lane; i < nal; i += kWave) {
-          const uint32_t e = L.alive[i];
-          L.eb[e] = pair_bound_res(sc, L.enr[e].x, L.enr[e].y, nb, rb);
-        }
     } else {
       const float* xc = base + size_t(L.eid[c]) * dim;
       for (uint32_t i = lane; i < nal; i += kWave) {
