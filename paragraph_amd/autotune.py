@@ -15,7 +15,8 @@ keeps the reference mapping.
 
 Method: one default build gives a realistic graph; then the search stage is
 timed on a sample of base rows for every search knob set, and the prune
-stage on the resulting rows for the prune variants.  The fastest set wins.
+stage on the search rows of all base rows for the prune variants.  The
+fastest set wins.
 Results are cached in a JSON file keyed by shape, precision and device.
 """
 import json
@@ -102,15 +103,14 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             log(f"autotune search {kn}: {1000 * t:.2f} ms")
         if t < best_t:
             best, best_t = dict(kn), t
-    # prune variants on real search rows of the sample (other rows empty):
-    # kept-major sweep or candidate-major scan, and for screened builds the
-    # split screen
-    rows = torch.empty((m, p.topk), **i32)
+    # prune variants on the search rows of every base row (the prune's cost
+    # depends on cache reuse across the whole node range, so a sample with
+    # empty rows elsewhere misranks them): kept-major sweep or
+    # candidate-major scan, and for screened builds the split screen
+    full = torch.empty((n, p.topk), **i32)
     with _native.Context(p) as ctx:
-        ctx.run_search(q, base_d, link, rows)
+        ctx.run_search(base_d, base_d, link, full)
         ctx.sync()
-    full = torch.full((n, p.topk), -1, **i32)
-    full[:m] = rows
     pr = torch.empty((n, p.M), **i32)
     pc = torch.empty(n, **i32)
     splits = (0, 0x80) if p.precision != _native.PG_FP32 else (0,)
