@@ -1694,6 +1694,31 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
   return 0;
 }
 
+// Ascending sort of the n keys a[0..n) in LDS (n a power of two): in
+// registers (64*R keys, shuffle exchanges, no barriers between stages) for n
+// up to 512, else the LDS bitonic sort.  The caller's last LDS writes must be
+// behind a barrier.
+template <int R>
+__device__ __forceinline__ void reg_sort_lds(uint64_t* a) {
+  const uint32_t lane = lane_id();
+  uint64_t v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = a[lane * R + r];
+  reg_bitonic_sort<R>(v);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[lane * R + r] = v[r];
+  __syncthreads();
+}
+
+__device__ __forceinline__ void sort_list(uint64_t* a, uint32_t n) {
+  if (n == 64) reg_sort_lds<1>(a);
+  else if (n == 128) reg_sort_lds<2>(a);
+  else if (n == 256) reg_sort_lds<4>(a);
+  else if (n == 512) reg_sort_lds<8>(a);
+  else bitonic_sort(a, n);
+}
+
 // HPL: top-list slices of 64 keys per lane (Km <= 64*HPL), CH: candidate
 // slices (Kp*Kd <= 64*CH).  Compile-time slice counts keep the merge free of
 // dead slices; larger lists fall back to a full sort per iteration.
@@ -1744,7 +1769,30 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
   ev.eval(cand, todo, kpkd, tail);
   unsigned long long ndist = kpkd;
   __syncthreads();
-  bitonic_sort(keys, lp2);
+  // The list after init is the first Km keys of sort(entry keys, Km tomb
+  // keys): the sorted entry keys below the tomb key, then tomb keys.  The
+  // tail is rewritten by the first iteration before anything reads it, so
+  // only the entry keys need sorting (in registers).  With max_iters == 0
+  // the initial list is the output, which needs the full sort.
+  constexpr int RS = CH <= 1 ? 1 : CH <= 2 ? 2 : 4;
+  if (max_iters > 0 && kpkd <= 64u * RS && Km <= 64u * RS) {
+    const uint64_t tomb_key = search_key(FLT_MAX, kTomb);
+    uint64_t v[RS];
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const uint32_t e = lane * RS + r;
+      v[r] = e < kpkd ? tail[e] : kPadKey;
+    }
+    reg_bitonic_sort<RS>(v);
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      const uint32_t e = lane * RS + r;
+      if (e < Km) keys[e] = v[r] < tomb_key ? v[r] : tomb_key;
+    }
+    __syncthreads();
+  } else {
+    bitonic_sort(keys, lp2);
+  }
   auto reset_table = [&]() {
     for (uint32_t i = lane; i < tsize; i += kWave) tab[i] = 0u;
     __syncthreads();
@@ -1837,7 +1885,7 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
         final_sort = merge_misses<(CH <= 4 ? 4 : 8), HPL>(
             keys, Km, tail, todo, ntodo, ms, last_allowed);
     }
-    if (final_sort) bitonic_sort(keys, lp2);
+    if (final_sort) sort_list(keys, lp2);
     ++total_iter;
     if (Eval::kExactVisited && ++iter == reset_iter) {
       reset_table();
