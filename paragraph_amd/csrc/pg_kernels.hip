@@ -1842,21 +1842,47 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
       reset_table();
       cache_count = Km;
     }
-    // neighbour ids of the expanded nodes (first Kd of each graph row)
-    for (uint32_t t = lane; t < kpkd; t += kWave) {
-      const uint32_t e = kd_shift < 32u ? (t >> kd_shift) : t / Kd;
-      const uint32_t j = t - e * Kd;
-      const uint32_t eid = explore[e];
-      cand[t] = eid == kTomb ? kTomb : graph[size_t(eid) * gdeg + j];
+    // neighbour ids of the expanded nodes (first Kd of each graph row),
+    // all slices in flight together, then the visited test-and-set slice
+    // by slice from registers; misses go to the todo list, hits get FLT_MAX
+    uint32_t nid[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint32_t t = lane + 64u * c;
+      nid[c] = kTomb;
+      if (t < kpkd) {
+        const uint32_t e = kd_shift < 32u ? (t >> kd_shift) : t / Kd;
+        const uint32_t j = t - e * Kd;
+        const uint32_t eid = explore[e];
+        if (eid != kTomb) nid[c] = graph[size_t(eid) * gdeg + j];
+      }
     }
-    __syncthreads();
-    // visited test-and-set; misses go to the todo list, hits get FLT_MAX
     uint32_t ntodo = 0;
-    for (uint32_t c0 = 0; c0 < kpkd; c0 += kWave) {
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint32_t t = lane + 64u * c;
+      bool miss = false;
+      if (t < kpkd) {
+        const uint32_t id = nid[c];
+        cand[t] = id;
+        if (id != kTomb) miss = visit_insert(tab, tsize, id);
+        if (!miss) tail[t] = search_key(FLT_MAX, id);
+      }
+      const uint64_t m = __ballot(miss);
+      if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = t;
+      ntodo += __popcll(m);
+    }
+    // candidate slots past 64*CH (Kp*Kd up to 4096), one slice at a time
+    for (uint32_t c0 = 64u * CH; c0 < kpkd; c0 += kWave) {
       const uint32_t t = c0 + lane;
       bool miss = false;
       if (t < kpkd) {
-        const uint32_t id = cand[t];
+        const uint32_t e = kd_shift < 32u ? (t >> kd_shift) : t / Kd;
+        const uint32_t j = t - e * Kd;
+        const uint32_t eid = explore[e];
+        const uint32_t id =
+            eid == kTomb ? kTomb : graph[size_t(eid) * gdeg + j];
+        cand[t] = id;
         if (id != kTomb) miss = visit_insert(tab, tsize, id);
         if (!miss) tail[t] = search_key(FLT_MAX, id);
       }

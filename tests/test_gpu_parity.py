@@ -243,6 +243,30 @@ def test_persistent_search_matches_oracle(cuda):
     np.testing.assert_array_equal(from_dev(out_d), ref)
 
 
+@pytest.mark.parametrize("Km,Kp,max_iters", [(64, 20, 100000), (100, 7, 1),
+                                              (100, 7, 3)])
+def test_search_wide_expansion_and_watchdog_match_oracle(Km, Kp, max_iters,
+                                                         cuda):
+    """Kp*Kd = 320 candidate slots (past the compile-time slices: the
+    generic gather / visited loop and the full-sort merge), and the watchdog
+    stopping after 1 and 3 iterations (the forced final sort)."""
+    import torch
+    w = workload("d200_m32_l100")
+    n = w["params"].n_base
+    rng = np.random.default_rng(11)
+    graph = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    base_d, graph_d = to_dev(w["base"], torch), to_dev(graph, torch)
+    p = _native.make_params(n, w["base"].shape[1], n, 32, Km, Kp=Kp,
+                            max_iters=max_iters)
+    out_d = torch.empty((n, p.topk), dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.run_search(base_d, base_d, graph_d, out_d)
+        ctx.sync()
+    ref = cpu_ref.search(w["base"], w["base"], graph, p.Km, p.Kp, p.Kd,
+                         p.topk, max_iters=max_iters)
+    np.testing.assert_array_equal(from_dev(out_d), ref)
+
+
 @pytest.mark.parametrize("prec", [_native.PG_FP32, _native.PG_BF16])
 def test_graph_replays_match_eager(prec, cuda):
     """hipGraph mode (tune bit 0x40): every replay, with eager work and
