@@ -1857,17 +1857,46 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
         if (eid != kTomb) nid[c] = graph[size_t(eid) * gdeg + j];
       }
     }
+    // Test-and-set of all slices in lockstep probe rounds, so that their
+    // LDS atomics are in flight together.  Slice c's CAS is issued before
+    // slice c+1's in every round and a duplicate id walks the same probe
+    // sequence, so an id in two slices is a miss in the earlier one, as
+    // with one slice after the other.
+    bool vmiss[CH], vact[CH];
+    uint32_t vslot[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const uint32_t t = lane + 64u * c;
+      vmiss[c] = false;
+      vact[c] = t < kpkd && nid[c] != kTomb;
+      vslot[c] = visit_hash(nid[c], tsize);
+      if (t < kpkd) cand[t] = nid[c];
+    }
+    while (true) {
+      bool any = false;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        if (vact[c]) {
+          const uint32_t old = atomicCAS(&tab[vslot[c]], 0u, nid[c] + 1u);
+          if (old == 0u) {
+            vmiss[c] = true;
+            vact[c] = false;
+          } else if (old == nid[c] + 1u) {
+            vact[c] = false;
+          } else {
+            vslot[c] = vslot[c] + 1u == tsize ? 0u : vslot[c] + 1u;
+          }
+        }
+        any = any || vact[c];
+      }
+      if (!any) break;
+    }
     uint32_t ntodo = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
       const uint32_t t = lane + 64u * c;
-      bool miss = false;
-      if (t < kpkd) {
-        const uint32_t id = nid[c];
-        cand[t] = id;
-        if (id != kTomb) miss = visit_insert(tab, tsize, id);
-        if (!miss) tail[t] = search_key(FLT_MAX, id);
-      }
+      const bool miss = vmiss[c];
+      if (t < kpkd && !miss) tail[t] = search_key(FLT_MAX, nid[c]);
       const uint64_t m = __ballot(miss);
       if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = t;
       ntodo += __popcll(m);
