@@ -109,15 +109,25 @@ def roofline_of(ctx, p, n, workload, precision):
     avg_ms = kd["ms"] / kd["launches"]
     bytes_per_launch = kd["bytes"] / kd["launches"]
     achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
-    kname = {"search": "search_kernel", "prune": "prune_kernel",
+    # the link-round prune runs as the kept-major sweep unless tune_flags
+    # bit 8 selects the candidate-major kernel
+    prune_k = "prune_kernel" if p.tune_flags & 0x100 else "prune_sweep_kernel"
+    kname = {"search": "search_kernel", "prune": prune_k,
              "merge": "merge_kernel", "reverse": "reverse_*"}.get(dom, dom)
+    traffic = profiled_traffic(kname, workload, precision)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(kname, workload, precision),
+                "traffic": traffic,
                 "kernel": kname,
                 "avg_launch_ms": round(avg_ms, 3),
                 "alg_bytes_per_launch": int(bytes_per_launch)}
+    if traffic:
+        # measured HBM rate: the profiled bytes per launch over the same
+        # launch time (algorithmic bytes above it come from L2 / MALL)
+        hbm = traffic / (avg_ms / 1000.0) / 1e9
+        roofline["hbm_gbs"] = round(hbm, 1)
+        roofline["hbm_frac"] = round(hbm / HBM_PEAK_GBS, 4)
     detail = {"stage_ms": {nm: round(x, 3) for nm, x in zip(names, stage_ms)},
               "stage_counters": {nm: c for nm, c in zip(names, counters)}}
     return roofline, detail
