@@ -137,10 +137,18 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, uint32_t m) {
   return (uint64_t(hi) << 32) | lo;
 }
 
-// Ascending bitonic sort of 64*R keys held in registers, element index
-// lane*R + r.  Cross-lane steps exchange through __shfl_xor (no LDS).
-template <int R>
-__device__ __forceinline__ void reg_bitonic_sort(uint64_t (&v)[R]) {
+__device__ __forceinline__ uint32_t shfl_xor_key(uint32_t v, uint32_t m) {
+  return __shfl_xor(v, int(m), 64);
+}
+__device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, uint32_t m) {
+  return shfl_xor_u64(v, m);
+}
+
+// Ascending bitonic sort of 64*R keys (uint32_t or uint64_t) held in
+// registers, element index lane*R + r.  Cross-lane steps exchange through
+// __shfl_xor (no LDS).
+template <int R, typename K>
+__device__ __forceinline__ void reg_bitonic_sort(K (&v)[R]) {
   const uint32_t lane = lane_id();
   constexpr uint32_t n = 64u * R;
 #pragma unroll
@@ -154,9 +162,9 @@ __device__ __forceinline__ void reg_bitonic_sort(uint64_t (&v)[R]) {
         for (int r = 0; r < R; ++r) {
           const uint32_t e = lane * R + r;
           const bool asc = (e & k) == 0;
-          const uint64_t o = shfl_xor_u64(v[r], lj);
-          const uint64_t mn = v[r] < o ? v[r] : o;
-          const uint64_t mx = v[r] < o ? o : v[r];
+          const K o = shfl_xor_key(v[r], lj);
+          const K mn = v[r] < o ? v[r] : o;
+          const K mx = v[r] < o ? o : v[r];
           v[r] = (lower == asc) ? mn : mx;
         }
       } else {
@@ -165,7 +173,7 @@ __device__ __forceinline__ void reg_bitonic_sort(uint64_t (&v)[R]) {
           if ((uint32_t(r) & j) == 0) {
             const uint32_t e = lane * R + r;
             const bool asc = (e & k) == 0;
-            const uint64_t a = v[r], b = v[r + j];
+            const K a = v[r], b = v[r + j];
             if ((a > b) == asc) {
               v[r] = b;
               v[r + j] = a;

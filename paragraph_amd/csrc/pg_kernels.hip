@@ -142,6 +142,32 @@ __device__ uint32_t rng_select(const float* __restrict__ base, uint32_t n_base,
   return nk;
 }
 
+// Ascending sort of the n keys a[0..n) in LDS (n a power of two): in
+// registers (64*R keys, shuffle exchanges, no barriers between stages) for n
+// up to 512, else the LDS bitonic sort.  The caller's last LDS writes must be
+// behind a barrier.
+template <int R, typename K>
+__device__ __forceinline__ void reg_sort_lds(K* a) {
+  const uint32_t lane = lane_id();
+  K v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = a[lane * R + r];
+  reg_bitonic_sort<R>(v);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[lane * R + r] = v[r];
+  __syncthreads();
+}
+
+template <typename K>
+__device__ __forceinline__ void sort_list(K* a, uint32_t n) {
+  if (n == 64) reg_sort_lds<1>(a);
+  else if (n == 128) reg_sort_lds<2>(a);
+  else if (n == 256) reg_sort_lds<4>(a);
+  else if (n == 512) reg_sort_lds<8>(a);
+  else bitonic_sort(a, n);
+}
+
 // Screened RNG (the BF16 path): the same decisions as rng_select, with most
 // pair distances taken from BF16 MFMA Gram tiles.  Candidates are resolved
 // in blocks of 16 (lane l%16 = block candidate): v_mfma_f32_16x16x32_bf16
@@ -436,7 +462,7 @@ __global__ __launch_bounds__(64) void prune_kernel(
     skey[j] = (uint64_t(ordered_bits(d)) << 32) | j;
   }
   __syncthreads();
-  bitonic_sort(skey, DP2);
+  sort_list(skey, DP2);
   unsigned long long pairs = 0, fbk = 0;
   uint32_t nk = 0;
   const uint32_t id0 = sid[uint32_t(skey[0])];
@@ -1193,7 +1219,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
     while (n2 < seg) n2 <<= 1;
     for (uint32_t j = lane; j < n2; j += kWave) buf[j] = j < seg ? srcs[s0 + j] : kTomb;
     __syncthreads();
-    if (seg > 1) bitonic_sort(buf, n2);
+    if (seg > 1) sort_list(buf, n2);
   } else {
     for (uint32_t j = lane; j < kSegChunk; j += kWave) buf[j] = srcs[s0 + j];
     __syncthreads();
@@ -1227,7 +1253,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
     for (uint32_t j = lane; j < deg; j += kWave) o[j] = kTomb;
     return;
   }
-  bitonic_sort(skey, 128u);
+  sort_list(skey, 128u);
   unsigned long long pairs = 0, fbk = 0;
   if (lane == 0) {
     kept[0] = sid[uint32_t(skey[0])];
@@ -1692,31 +1718,6 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
   }
   __syncthreads();
   return 0;
-}
-
-// Ascending sort of the n keys a[0..n) in LDS (n a power of two): in
-// registers (64*R keys, shuffle exchanges, no barriers between stages) for n
-// up to 512, else the LDS bitonic sort.  The caller's last LDS writes must be
-// behind a barrier.
-template <int R>
-__device__ __forceinline__ void reg_sort_lds(uint64_t* a) {
-  const uint32_t lane = lane_id();
-  uint64_t v[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) v[r] = a[lane * R + r];
-  reg_bitonic_sort<R>(v);
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < R; ++r) a[lane * R + r] = v[r];
-  __syncthreads();
-}
-
-__device__ __forceinline__ void sort_list(uint64_t* a, uint32_t n) {
-  if (n == 64) reg_sort_lds<1>(a);
-  else if (n == 128) reg_sort_lds<2>(a);
-  else if (n == 256) reg_sort_lds<4>(a);
-  else if (n == 512) reg_sort_lds<8>(a);
-  else bitonic_sort(a, n);
 }
 
 // HPL: top-list slices of 64 keys per lane (Km <= 64*HPL), CH: candidate
