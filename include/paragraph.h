@@ -90,7 +90,10 @@ typedef struct pg_params {
                            (tighter bound, fewer exact fallbacks); bit 8
                            (0x100): prune the link rows candidate by
                            candidate (the reference's loop order) instead of
-                           kept-major sweeps (same rows) */
+                           kept-major sweeps (same rows); bit 9 (0x200):
+                           the persistent search takes its queries in a
+                           locality order (nodes grouped by nearest pivot
+                           row, one queue segment per XCD; same rows) */
   uint32_t reserved[7];
 } pg_params;
 

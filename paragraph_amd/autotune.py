@@ -4,6 +4,7 @@ Knobs (pg_params.tune_flags and table_size):
   - gather batch: FP32 search, candidate pairs in flight (4, 8 or 16);
     screened search, 16-candidate MFMA groups in flight (1 or 2);
   - persistent search grid on / off;
+  - locality order of the persistent search's queries on / off;
   - split-BF16 screen in the screened prune / merge on / off;
   - link-row prune: kept-major sweep or candidate-major scan;
   - visited-table size (FP32: exact set, any size above the bound gives the
@@ -14,8 +15,10 @@ whose recall@10 stays within a tolerance of the FP32 build's; the FP32 path
 keeps the reference mapping.
 
 Method: one default build gives a realistic graph; then the search stage is
-timed on a sample of base rows for every search knob set, and the prune
-stage on the search rows of all base rows for the prune variants.  The
+timed on a sample of base rows for every search knob set, the locality
+order on all base rows (its gain is cache reuse across the whole query
+range), and the prune stage on the search rows of all base rows for the
+prune variants.  The
 fastest set wins.
 Results are cached in a JSON file keyed by shape, precision and device.
 """
@@ -103,6 +106,32 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             log(f"autotune search {kn}: {1000 * t:.2f} ms")
         if t < best_t:
             best, best_t = dict(kn), t
+    # grid shape on all base rows, with the best sampled gather / table
+    # knobs: one workgroup per query, the persistent grid, and the
+    # persistent grid taking its queries in the locality order (their gains
+    # are cache reuse and load balance across the whole query range, which
+    # a sample does not show)
+    if best is not None:
+        full_t = {}
+        for grid in (0, 0x20, 0x220):
+            pp = _native.PgParams.from_buffer_copy(p)
+            pp.tune_flags = ((p.tune_flags & ~0x23F)
+                             | (best["tune_flags"] & 0x1F) | grid)
+            pp.table_size = best["table_size"]
+            out = torch.empty((n, pp.topk), **i32)
+            with _native.Context(pp) as ctx:
+                full_t[grid] = _time(lambda: (ctx.run_search(
+                    base_d, base_d, link, out), ctx.sync()))
+            del out
+            report["search"].append({"tune_flags": int(pp.tune_flags),
+                                     "table_size": best["table_size"],
+                                     "rows": n,
+                                     "ms": round(1000 * full_t[grid], 3)})
+            if log:
+                log(f"autotune search grid={grid:#x} (all rows): "
+                    f"{1000 * full_t[grid]:.2f} ms")
+        g = min(full_t, key=full_t.get)
+        best["tune_flags"] = (best["tune_flags"] & ~0x220) | g
     # prune variants on the search rows of every base row (the prune's cost
     # depends on cache reuse across the whole node range, so a sample with
     # empty rows elsewhere misranks them): kept-major sweep or
@@ -145,7 +174,7 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
 def apply(p, best):
     """Copy of p with the tuned knobs."""
     pp = _native.PgParams.from_buffer_copy(p)
-    pp.tune_flags = (p.tune_flags & ~0x1BF) | best["tune_flags"]
+    pp.tune_flags = (p.tune_flags & ~0x3BF) | best["tune_flags"]
     pp.table_size = best["table_size"]
     return pp
 

@@ -61,7 +61,10 @@ struct pg_ctx {
   float gam_a = 0.0f, gam_c = 0.0f;  // plain screen: MFMA / chain error
   unsigned long long* stats = nullptr;
   unsigned long long* stage_cnt = nullptr;  // (kMaxStages+1) x 16 snapshots
-  uint32_t* queue = nullptr;                // persistent-search work counter
+  uint32_t* queue = nullptr;                // persistent-search work counters
+  uint32_t* order = nullptr;   // locality order of the search queries
+  uint32_t* ogroup = nullptr;  // its pivot group per node
+  uint32_t* ocnt = nullptr;    // its group counters
   bool capturing = false;                   // recording a hipGraph
   hipGraph_t graph = nullptr;               // captured pipeline
   hipGraphExec_t graph_exec = nullptr;
@@ -139,10 +142,14 @@ uint32_t effective_table_size(const pg_params& p) {
 //             its sweep on exact FP32 chains
 //   bit 8     link-row prune candidate by candidate (the reference's loop
 //             order) instead of the kept-major sweep
+//   bit 9     persistent search takes its queries in a locality order
+//             (nodes grouped by nearest pivot row, one queue segment per
+//             XCD); same rows
 constexpr uint32_t kTunePersistent = 0x20u;
 constexpr uint32_t kTuneGraph = 0x40u;
 constexpr uint32_t kTuneSplit = 0x80u;
 constexpr uint32_t kTuneScanPrune = 0x100u;
+constexpr uint32_t kTuneOrder = 0x200u;
 uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
 
 // Bound factor of the BF16 screen against the exact FP32 chain:
@@ -221,6 +228,21 @@ int prepare_screen(pg_ctx* c, const float* d_base) {
   return PG_OK;
 }
 
+// Locality order of `n` query rows for the persistent search (tune_flags
+// bit 9): *out = the order, or nullptr when the order is off, the grid is
+// not persistent or n exceeds the order's buffers (n_base rows).
+int search_order(pg_ctx* c, const float* d_rows, uint32_t n,
+                 const uint32_t** out) {
+  *out = nullptr;
+  if (!c->order || !(c->p.tune_flags & kTunePersistent) || n == 0 ||
+      n > c->p.n_base)
+    return PG_OK;
+  PG_HIP(c, pg::launch_locality_order(d_rows, n, c->p.dim, c->ogroup, c->ocnt,
+                                      c->order, c->stream));
+  *out = c->order;
+  return PG_OK;
+}
+
 int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
                  uint32_t* d_proj, uint32_t* d_link) {
   const pg_params& p = c->p;
@@ -247,6 +269,8 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   PG_TRY(record(c, ev++));
   const uint32_t* graph = d_proj;
   uint32_t gdeg = p.proj_deg;
+  const uint32_t* sorder = nullptr;
+  PG_TRY(search_order(c, d_base, N, &sorder));
   // Screened paths: the search emits its first `rerank` entries (ordered by
   // screened distance) and the prune re-ranks exactly those in FP32.
   const uint32_t width =
@@ -262,7 +286,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
                                 c->res, c->stats,
                                 (p.tune_flags & kTunePersistent) ? c->queue
                                                                  : nullptr,
-                                s));
+                                sorder, s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, width, nullptr,
                                nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
@@ -490,7 +514,11 @@ pg_ctx* pg_create(int device, const pg_params* p) {
         dev_alloc(c, &c->scale8, N, "scale8"))) ||
       dev_alloc(c, &c->stats, kStatCount, "stats") ||
       dev_alloc(c, &c->stage_cnt, (kMaxStages + 1) * 16ull, "stage_cnt") ||
-      dev_alloc(c, &c->queue, 1, "queue"))
+      dev_alloc(c, &c->queue, pg::kQueueSegs, "queue") ||
+      ((p->tune_flags & kTuneOrder) &&
+       (dev_alloc(c, &c->order, N, "order") ||
+        dev_alloc(c, &c->ogroup, N, "order_group") ||
+        dev_alloc(c, &c->ocnt, 2ull * pg::kOrderGroups, "order_cnt"))))
     return fail(0);
   if (pg::launch_fill_u64(c->stats, kStatCount, 0ull, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -524,6 +552,9 @@ void pg_destroy(pg_ctx* c) {
   dev_free(c->stats);
   dev_free(c->stage_cnt);
   dev_free(c->queue);
+  dev_free(c->order);
+  dev_free(c->ogroup);
+  dev_free(c->ocnt);
   dev_free(c->h_base_dev);
   dev_free(c->h_gt_dev);
   dev_free(c->h_proj_dev);
@@ -722,6 +753,8 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
                    graph_deg);
   PG_HIP(c, hipSetDevice(c->device));
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
+  const uint32_t* sorder = nullptr;
+  PG_TRY(search_order(c, d_queries, n_query, &sorder));
   PG_HIP(c, pg::launch_search(d_queries, n_query, d_base,
                               c->base8 ? nullptr : c->base16, c->base8,
                               c->scale8, c->dpad,
@@ -732,7 +765,7 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
                               c->stats,
                               (p.tune_flags & kTunePersistent) ? c->queue
                                                                : nullptr,
-                              c->stream));
+                              sorder, c->stream));
   return PG_OK;
 }
 

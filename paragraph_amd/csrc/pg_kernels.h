@@ -17,6 +17,14 @@ struct ScreenArgs {
   float gam_a, gam_c;      // plain screen: accumulation / chain factors
 };
 
+// Persistent search queue: one counter per segment of the locality order
+// (one segment per XCD).
+constexpr uint32_t kQueueSegs = 8;
+// Locality order: groups per pivot row and the workspace it needs
+// (group id per node, then 2 x kOrderGroups counters).
+constexpr uint32_t kOrderGroups = 256;
+constexpr uint32_t kOrderDims = 64;
+
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n,
                            unsigned long long v, hipStream_t s);
@@ -52,7 +60,18 @@ hipError_t launch_prepare_screen(const float* base, uint32_t n, uint32_t dim,
 hipError_t launch_prepare_fp8(const float* base, uint32_t n, uint32_t dim,
                               uint32_t dpad, void* out8, float* scale,
                               hipStream_t s);
+// Locality order of the nodes: node i joins the group of its nearest pivot
+// row (pivots = rows (g * n) / kOrderGroups, squared L2 over the first
+// min(dim, kOrderDims) elements, ties to the lower pivot), and `order` lists
+// the nodes group by group.  Only the processing order of the search
+// queries depends on it, never a result.  group: n words; cnt: 2 x
+// kOrderGroups words.
+hipError_t launch_locality_order(const float* base, uint32_t n, uint32_t dim,
+                                 uint32_t* group, uint32_t* cnt,
+                                 uint32_t* order, hipStream_t s);
 // base8 != nullptr: FP8-screened search over per-row-scaled e4m3 rows.
+// order != nullptr (persistent grid only): queries are taken in that order,
+// kQueueSegs segments with a counter each in queue[0 .. kQueueSegs).
 hipError_t launch_search(const float* queries, uint32_t n_query,
                          const float* base, const void* base16,
                          const void* base8, const float* scale8, uint32_t dpad,
@@ -63,7 +82,7 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t max_iters, uint32_t table_size,
                          uint32_t gather_batch, uint32_t* out,
                          unsigned long long* stats, uint32_t* queue,
-                         hipStream_t s);
+                         const uint32_t* order, hipStream_t s);
 
 // Pair distances: chain != 0 gives the FP32 prune chain; else the search
 // evaluator selected like launch_search (k <= 1024).

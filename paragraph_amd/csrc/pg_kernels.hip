@@ -1732,7 +1732,8 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     uint32_t Kp, uint32_t Kd, uint32_t topk, uint32_t reset_iter,
     uint32_t entry_seed, uint32_t max_iters, uint32_t tsize, uint32_t lp2,
     uint32_t kd_shift, uint32_t* __restrict__ out,
-    unsigned long long* __restrict__ stats, uint32_t* __restrict__ queue) {
+    unsigned long long* __restrict__ stats, uint32_t* __restrict__ queue,
+    const uint32_t* __restrict__ order) {
   extern __shared__ __align__(16) uint8_t smem[];
   const uint32_t kpkd = Kp * Kd;
   ev.qs = reinterpret_cast<bf16x8*>(smem);
@@ -1747,9 +1748,37 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
   const uint32_t lane = lane_id();
   // One query per workgroup, or (queue != nullptr) a persistent grid whose
   // waves take query ids from a device counter until it runs past n_query.
+  // With a locality order, the queries are taken in that order (nodes
+  // grouped by nearest pivot row, so that concurrent searches gather
+  // overlapping rows), split into kQueueSegs contiguous segments with a
+  // counter each.  A wave starts on segment blockIdx % kQueueSegs, which is
+  // the XCD its workgroup is dispatched to (round-robin), and moves to the
+  // next segment once its own is drained; segments only drain, so a wave
+  // passes each empty segment once and the loop ends.
+  const uint32_t seg_len = (n_query + kQueueSegs - 1) / kQueueSegs;
+  uint32_t seg = blockIdx.x % kQueueSegs, segs_left = kQueueSegs;
   auto next_query = [&]() -> uint32_t {
-    uint32_t v = 0;
-    if (lane == 0) v = atomicAdd(queue, 1u);
+    uint32_t v = n_query;
+    if (lane == 0) {
+      if (order == nullptr) {
+        v = atomicAdd(queue, 1u);
+      } else {
+        while (segs_left > 0) {
+          const uint32_t lo = seg * seg_len;
+          const uint32_t len =
+              lo >= n_query ? 0u : (n_query - lo < seg_len ? n_query - lo : seg_len);
+          if (len > 0) {
+            const uint32_t i = atomicAdd(&queue[seg], 1u);
+            if (i < len) {
+              v = order[lo + i];
+              break;
+            }
+          }
+          seg = seg + 1u == kQueueSegs ? 0u : seg + 1u;
+          --segs_left;
+        }
+      }
+    }
     return __shfl(v, 0, 64);
   };
   uint32_t q = queue ? next_query() : blockIdx.x;
@@ -2145,7 +2174,8 @@ static hipError_t launch_search_eval(
     const uint32_t* graph, uint32_t gdeg, uint32_t Km, uint32_t Kp,
     uint32_t Kd, uint32_t topk, uint32_t reset_iter, uint32_t entry_seed,
     uint32_t max_iters, uint32_t table_size, uint32_t* out,
-    unsigned long long* stats, uint32_t* queue, hipStream_t s) {
+    unsigned long long* stats, uint32_t* queue, const uint32_t* order,
+    hipStream_t s) {
   uint32_t lp2 = 64;
   while (lp2 < Km + Kp * Kd) lp2 <<= 1;
   const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp) +
@@ -2176,13 +2206,14 @@ static hipError_t launch_search_eval(
     if (cus < 1) cus = 1;
     const uint64_t resident = uint64_t(per_cu) * uint64_t(cus);
     grid = uint32_t(resident < n_query ? resident : n_query);
-    hipError_t e = launch_fill_u32(queue, 1, 0u, s);
+    hipError_t e = launch_fill_u32(queue, kQueueSegs, 0u, s);
     if (e != hipSuccess) return e;
   }
 #define PG_SEARCH_LAUNCH(H, C)                                              \
   search_kernel<Eval, H, C><<<grid, 64, smem, s>>>(                         \
       ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter, \
-      entry_seed, max_iters, table_size, lp2, kd_shift, out, stats, queue)
+      entry_seed, max_iters, table_size, lp2, kd_shift, out, stats, queue,  \
+      queue ? order : nullptr)
   if (slices == 2) PG_SEARCH_LAUNCH(2, 2);
   else if (slices == 3) PG_SEARCH_LAUNCH(3, 3);
   else PG_SEARCH_LAUNCH(4, 4);
@@ -2199,13 +2230,13 @@ static hipError_t launch_f32(const float* base, uint32_t dim,
                              uint32_t entry_seed, uint32_t max_iters,
                              uint32_t table_size, uint32_t* out,
                              unsigned long long* stats, uint32_t* queue,
-                             hipStream_t s) {
+                             const uint32_t* order, hipStream_t s) {
   EvalF32<TQ, U> ev;
   ev.base = base;
   ev.dim = dim;
   return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
                             Kd, topk, reset_iter, entry_seed, max_iters,
-                            table_size, out, stats, queue, s);
+                            table_size, out, stats, queue, order, s);
 }
 
 template <int NCH, int G>
@@ -2217,7 +2248,7 @@ static hipError_t launch_bf16(const void* base16, uint32_t dim, uint32_t dpad,
                               uint32_t entry_seed, uint32_t max_iters,
                               uint32_t table_size, uint32_t* out,
                               unsigned long long* stats, uint32_t* queue,
-                              hipStream_t s) {
+                              const uint32_t* order, hipStream_t s) {
   EvalBF16<NCH, G> ev;
   ev.base16 = reinterpret_cast<const __bf16*>(base16);
   ev.dim = dim;
@@ -2225,7 +2256,7 @@ static hipError_t launch_bf16(const void* base16, uint32_t dim, uint32_t dpad,
   ev.nch = dpad / 32;
   return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
                             Kd, topk, reset_iter, entry_seed, max_iters,
-                            table_size, out, stats, queue, s);
+                            table_size, out, stats, queue, order, s);
 }
 
 template <int NCH, int G>
@@ -2237,7 +2268,8 @@ static hipError_t launch_fp8(const void* base8, const float* scale8,
                              uint32_t reset_iter, uint32_t entry_seed,
                              uint32_t max_iters, uint32_t table_size,
                              uint32_t* out, unsigned long long* stats,
-                             uint32_t* queue, hipStream_t s) {
+                             uint32_t* queue, const uint32_t* order,
+                             hipStream_t s) {
   EvalFP8<NCH, G> ev;
   ev.base8 = reinterpret_cast<const uint8_t*>(base8);
   ev.scale8 = scale8;
@@ -2246,7 +2278,7 @@ static hipError_t launch_fp8(const void* base8, const float* scale8,
   ev.nch = dpad / 32;
   return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
                             Kd, topk, reset_iter, entry_seed, max_iters,
-                            table_size, out, stats, queue, s);
+                            table_size, out, stats, queue, order, s);
 }
 
 hipError_t launch_search(const float* queries, uint32_t n_query,
@@ -2259,11 +2291,11 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t max_iters, uint32_t table_size,
                          uint32_t gather_batch, uint32_t* out,
                          unsigned long long* stats, uint32_t* queue,
-                         hipStream_t s) {
+                         const uint32_t* order, hipStream_t s) {
   if (n_query == 0) return hipSuccess;
 #define PG_ARGS                                                             \
   queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,      \
-      entry_seed, max_iters, table_size, out, stats, queue, s
+      entry_seed, max_iters, table_size, out, stats, queue, order, s
   if (base8 != nullptr) {
     const uint32_t nch = dpad / 32;
     const bool two = gather_batch >= 2;
@@ -2307,6 +2339,144 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
 #undef PG_F32_U
 #undef PG_ARGS
   return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------- locality order
+// Groups nodes by nearest pivot row so that a persistent search grid can
+// take queries whose neighbourhoods overlap at the same time (their row
+// gathers then hit the XCD's L2 and the MALL instead of HBM).  The order
+// never changes a result: every query's search is independent of when it
+// runs.
+
+constexpr uint32_t kOrderChunk = 4096;  // nodes per block in count / scatter
+
+// Nearest pivot per node: one thread per node holds its first ds elements;
+// the pivots are staged through LDS kTile at a time.  Score |p|^2 - 2 x.p
+// (squared L2 minus |x|^2), ties to the lower pivot.
+__global__ __launch_bounds__(256) void pivot_group_kernel(
+    const float* __restrict__ base, uint32_t n, uint32_t dim, uint32_t ds,
+    uint32_t npiv, uint32_t* __restrict__ group) {
+  constexpr uint32_t kTile = 128;
+  __shared__ float piv[kTile][kOrderDims];
+  __shared__ float pn[kTile];
+  const uint32_t t = threadIdx.x;
+  const uint32_t i = blockIdx.x * 256u + t;
+  const float* row = base + size_t(i < n ? i : 0u) * dim;
+  float x[kOrderDims];
+#pragma unroll
+  for (uint32_t d = 0; d < kOrderDims; ++d) x[d] = d < ds ? row[d] : 0.0f;
+  float best = FLT_MAX;
+  uint32_t bg = 0;
+  for (uint32_t p0 = 0; p0 < npiv; p0 += kTile) {
+    __syncthreads();
+    for (uint32_t e = t; e < kTile * kOrderDims; e += 256u) {
+      const uint32_t pp = e / kOrderDims, d = e % kOrderDims;
+      const uint32_t g = p0 + pp;
+      float v = 0.0f;
+      if (g < npiv && d < ds)
+        v = base[size_t(uint64_t(g) * n / npiv) * dim + d];
+      piv[pp][d] = v;
+    }
+    __syncthreads();
+    if (t < kTile) {
+      float a = 0.0f;
+      for (uint32_t d = 0; d < kOrderDims; ++d)
+        a = __builtin_fmaf(piv[t][d], piv[t][d], a);
+      pn[t] = a;
+    }
+    __syncthreads();
+    const uint32_t pe = npiv - p0 < kTile ? npiv - p0 : kTile;
+    for (uint32_t pp = 0; pp < pe; ++pp) {
+      float dot = 0.0f;
+#pragma unroll
+      for (uint32_t d = 0; d < kOrderDims; ++d)
+        dot = __builtin_fmaf(x[d], piv[pp][d], dot);
+      const float sc = pn[pp] - 2.0f * dot;
+      if (sc < best) {
+        best = sc;
+        bg = p0 + pp;
+      }
+    }
+  }
+  if (i < n) group[i] = bg;
+}
+
+// Group sizes: LDS histogram per block of kOrderChunk nodes, one global
+// atomic per non-empty group.
+__global__ __launch_bounds__(256) void group_count_kernel(
+    const uint32_t* __restrict__ group, uint32_t n, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t h[kOrderGroups];
+  const uint32_t t = threadIdx.x;
+  h[t] = 0u;
+  __syncthreads();
+  const uint32_t i0 = blockIdx.x * kOrderChunk;
+#pragma unroll
+  for (uint32_t k = 0; k < kOrderChunk / 256u; ++k) {
+    const uint32_t i = i0 + t + 256u * k;
+    if (i < n) atomicAdd(&h[group[i]], 1u);
+  }
+  __syncthreads();
+  if (h[t]) atomicAdd(&cnt[t], h[t]);
+}
+
+// Scatter: each block ranks its nodes per group in LDS, reserves one run
+// per group with a global atomic (cur), and writes the node ids at
+// (group offset + run + rank).  The order inside a group is not fixed.
+__global__ __launch_bounds__(256) void group_scatter_kernel(
+    const uint32_t* __restrict__ group, uint32_t n,
+    const uint32_t* __restrict__ cnt, uint32_t* __restrict__ cur,
+    uint32_t* __restrict__ order) {
+  __shared__ uint32_t off[kOrderGroups], lc[kOrderGroups], lb[kOrderGroups];
+  const uint32_t t = threadIdx.x;
+  const uint32_t mine = cnt[t];
+  off[t] = mine;
+  lc[t] = 0u;
+  __syncthreads();
+  for (uint32_t d = 1; d < kOrderGroups; d <<= 1) {
+    const uint32_t v = t >= d ? off[t - d] : 0u;
+    __syncthreads();
+    off[t] += v;
+    __syncthreads();
+  }
+  constexpr uint32_t kPer = kOrderChunk / 256u;
+  const uint32_t i0 = blockIdx.x * kOrderChunk;
+  uint32_t g[kPer], r[kPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = i0 + t + 256u * k;
+    g[k] = 0u;
+    r[k] = 0u;
+    if (i < n) {
+      g[k] = group[i];
+      r[k] = atomicAdd(&lc[g[k]], 1u);
+    }
+  }
+  __syncthreads();
+  lb[t] = (off[t] - mine) + (lc[t] ? atomicAdd(&cur[t], lc[t]) : 0u);
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = i0 + t + 256u * k;
+    if (i < n) order[lb[g[k]] + r[k]] = i;
+  }
+}
+
+hipError_t launch_locality_order(const float* base, uint32_t n, uint32_t dim,
+                                 uint32_t* group, uint32_t* cnt,
+                                 uint32_t* order, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  static_assert(kOrderGroups == 256, "one thread per group in count/scatter");
+  const uint32_t npiv = n < kOrderGroups ? n : kOrderGroups;
+  const uint32_t ds = dim < kOrderDims ? dim : kOrderDims;
+  hipError_t e = launch_fill_u32(cnt, 2ull * kOrderGroups, 0u, s);
+  if (e != hipSuccess) return e;
+  pivot_group_kernel<<<(n + 255u) / 256u, 256, 0, s>>>(base, n, dim, ds, npiv,
+                                                       group);
+  const uint32_t nb = (n + kOrderChunk - 1) / kOrderChunk;
+  group_count_kernel<<<nb, 256, 0, s>>>(group, n, cnt);
+  group_scatter_kernel<<<nb, 256, 0, s>>>(group, n, cnt, cnt + kOrderGroups,
+                                          order);
+  return hipGetLastError();
 }
 
 // Screened-path preparation, one wave per row: the BF16 copy of the row

@@ -243,6 +243,64 @@ def test_persistent_search_matches_oracle(cuda):
     np.testing.assert_array_equal(from_dev(out_d), ref)
 
 
+@pytest.mark.parametrize("n_query", [2000, 1003, 5, 1])
+def test_locality_order_search_matches_oracle(n_query, cuda):
+    """The persistent grid taking its queries in the locality order (tune
+    bit 0x200: nearest-pivot groups, one queue segment per XCD) computes
+    every query exactly as the oracle, for query counts that leave some of
+    the 8 segments short or empty; rows are pre-filled, so a query the
+    queue skipped would show."""
+    import torch
+    w = workload("d200_m32_l100")
+    n = w["params"].n_base
+    rng = np.random.default_rng(6)
+    graph = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    queries = np.ascontiguousarray(w["base"][rng.permutation(n)[:n_query]])
+    base_d, graph_d = to_dev(w["base"], torch), to_dev(graph, torch)
+    q_d = to_dev(queries, torch)
+    p = _native.make_params(n, w["base"].shape[1], n, 32, 100,
+                            tune_flags=0x20 | 0x200)
+    out_d = torch.full((n_query, p.topk), -7, dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        for _ in range(2):  # the order is rebuilt per call
+            ctx.run_search(q_d, base_d, graph_d, out_d)
+            ctx.sync()
+    ref = cpu_ref.search(queries, w["base"], graph, p.Km, p.Kp, p.Kd, p.topk)
+    np.testing.assert_array_equal(from_dev(out_d), ref)
+
+
+@pytest.mark.parametrize("prec", [_native.PG_FP32, _native.PG_BF16])
+def test_locality_order_build_equals_plain_build(prec, cuda):
+    """A whole build with the locality order (eager, and captured in a
+    hipGraph and replayed) gives the same rows as without it; the FP32
+    rows also equal the oracle."""
+    import torch
+    w = workload("d200_m32_l100")
+    p0 = w["params"]
+    n = p0.n_base
+    base_d, gt_d = to_dev(w["base"], torch), to_dev(w["gt_u32"], torch)
+    outs = []
+    for tf in (0x20, 0x20 | 0x200, 0x20 | 0x200 | 0x40):
+        p = _native.make_params(n, p0.dim, n, p0.M, p0.Km, precision=prec,
+                                tune_flags=tf)
+        proj_d = torch.full((n, 16), -7, dtype=torch.int32, device=cuda)
+        link_d = torch.full((n, p0.M), -7, dtype=torch.int32, device=cuda)
+        with _native.Context(p) as ctx:
+            for _ in range(2 if tf & 0x40 else 1):
+                torch.cuda.synchronize()
+                ctx.stage_device(base_d, gt_d, proj_d, link_d)
+                ctx.wait()
+        outs.append((from_dev(proj_d), from_dev(link_d)))
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o[0], outs[0][0])
+        np.testing.assert_array_equal(o[1], outs[0][1])
+    if prec == _native.PG_FP32:
+        proj_o, link_o = cpu_ref.gpu_stage(orc_params(p0), w["base"],
+                                           w["gt_u32"])
+        np.testing.assert_array_equal(outs[1][0], proj_o)
+        np.testing.assert_array_equal(outs[1][1], link_o)
+
+
 @pytest.mark.parametrize("Km,Kp,max_iters", [(64, 20, 100000), (100, 7, 1),
                                               (100, 7, 3)])
 def test_search_wide_expansion_and_watchdog_match_oracle(Km, Kp, max_iters,
