@@ -186,6 +186,12 @@ def main():
                     help="gt = exact top-128 within shards of this many base "
                          "rows (default: exact over all rows up to 3M base "
                          "rows, 200000-row shards above)")
+    ap.add_argument("--end-to-end", action="store_true",
+                    help="also time the whole index build of the value "
+                         "path: GPU stage with the host bipartite stage "
+                         "(MatchSup rounds + FusionNN) on a CPU thread "
+                         "beside it, then FusionFinal and the entry point "
+                         "(reported in extra, never as value)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
@@ -366,6 +372,14 @@ def main():
                            hlink, hp["link"].cpu().numpy().view(np.uint32)))}
         log(f"host window (H2D + stage + D2H): {host_window}")
 
+    # ---- end-to-end window: the reference's whole build
+    # (tests/test_t2iend.cu:850-1243), inputs in host memory for the host
+    # stages and in HBM for the GPU stage
+    end_to_end = None
+    if args.end_to_end:
+        end_to_end = time_end_to_end(results, precs, w, gt_u32, base_d, gt_d,
+                                     n, M, log)
+
     # ---- CPU baseline: the oracle on a bounded node sample
     cpu = None
     if not args.no_cpu_baseline:
@@ -396,6 +410,8 @@ def main():
         extra["distance_error_vs_fp64"] = dist_err
     if host_window is not None:
         extra["host_window"] = host_window
+    if end_to_end is not None:
+        extra["end_to_end"] = end_to_end
     for sp in screened:
         if "fp32" not in results:
             break
@@ -454,6 +470,57 @@ def main():
     print(json.dumps(line), flush=True)
     for r in results.values():
         r["ctx"].close()
+
+
+def time_end_to_end(results, precs, w, gt_u32, base_d, gt_d, n, M, log):
+    """One whole index build of the value path: the GPU stage (device
+    entry) with the host bipartite stage on a CPU thread beside it, then
+    FusionFinal and the entry point on the host."""
+    import threading
+
+    import torch
+    from paragraph_amd import host_stage
+    prec = next((x for x in precs if x != "fp32"), "fp32")
+    ctx = results[prec]["ctx"]
+    base_h = np.ascontiguousarray(w["base"])
+    hp = host_stage.make_params(n, base_h.shape[1], gt_u32.shape[0],
+                                gt_k=gt_u32.shape[1])
+    proj_d = torch.empty((n, ctx.params.proj_deg), dtype=torch.int32,
+                         device=base_d.device)
+    link_d = torch.empty((n, M), dtype=torch.int32, device=base_d.device)
+    torch.cuda.synchronize()
+    bip, t_host = {}, {}
+
+    def host_work():
+        t0 = time.perf_counter()
+        bip["csr"] = host_stage.bipartite(hp, base_h, gt_u32)
+        t_host["bipartite"] = time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=host_work)
+    th.start()
+    ctx.stage_device(base_d, gt_d, proj_d, link_d)
+    ctx.wait()
+    t_gpu = time.perf_counter() - t0
+    proj = proj_d.cpu().numpy().view(np.uint32)
+    link = link_d.cpu().numpy().view(np.uint32)
+    th.join()
+    t1 = time.perf_counter()
+    ep = host_stage.entry_point(hp, gt_u32)
+    off, ids = host_stage.fusion_final(hp, base_h, *bip["csr"], link, proj)
+    t_fin = time.perf_counter() - t1
+    total = time.perf_counter() - t0
+    lens = np.diff(off.astype(np.int64))
+    r = {"precision": prec, "ms": round(1000 * total, 1),
+         "base_points_per_s": round(n / total, 1),
+         "gpu_stage_ms": round(1000 * t_gpu, 1),
+         "host_bipartite_ms": round(1000 * t_host["bipartite"], 1),
+         "fusion_final_and_ep_ms": round(1000 * t_fin, 1),
+         "host_threads": host_stage.default_threads(),
+         "entry_point": ep, "final_degree_mean": round(float(lens.mean()), 2),
+         "final_degree_max": int(lens.max())}
+    log(f"end-to-end build ({prec}): {r}")
+    return r
 
 
 def distance_error(ctxs, w, base_d, n_sample=2000):

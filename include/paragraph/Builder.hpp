@@ -15,9 +15,19 @@
 #include <utility>
 #include <vector>
 
+#include <thread>
+
 #include "../paragraph.h"
+#include "../paragraph_host.h"
 
 namespace Gbuilder {
+
+// The final index: entry point and one list per base as CSR.
+struct Index {
+  uint32_t ep = 0;
+  std::vector<uint32_t> off;  // N + 1
+  std::vector<uint32_t> ids;
+};
 
 struct BuildResult {
   std::vector<uint32_t> projection;  // N x proj_deg, tomb-padded
@@ -82,6 +92,60 @@ class Builder {
     }
     pg_destroy(ctx);
     return r;
+  }
+
+  // The reference's whole build (tests/test_t2iend.cu:850-1258): the GPU
+  // stage, with the host bipartite stage (MatchSup rounds + FusionNN) on a
+  // CPU thread beside it, then FusionFinal and the entry point.  `hp`
+  // defaults to the reference's host values (pg_host_params_init).  Links
+  // libparagraph_hip and libparagraph_host.
+  auto BuildIndex(uint32_t M, uint32_t L, int device = 0,
+                  pg_params const* params = nullptr,
+                  pg_host_params const* hp = nullptr) -> Index {
+    const uint32_t n = uint32_t(base_.size() / dim_);
+    const uint32_t q = uint32_t(gt_ids_.size() / gt_k_);
+    pg_host_params h;
+    if (hp) {
+      h = *hp;
+    } else {
+      pg_host_params_init(&h, n, dim_, q);
+      h.gt_k = gt_k_;
+    }
+    const float* base = reinterpret_cast<const float*>(base_.data());
+    std::vector<uint32_t> bi_off(size_t(n) + 1), bi_ids(size_t(n) * 2 * h.M_nn);
+    int host_rc = PG_OK;
+    std::thread host([&] {
+      host_rc = pg_host_bipartite(&h, base, gt_ids_.data(), bi_off.data(),
+                                  bi_ids.data());
+    });
+    BuildResult r;
+    try {
+      r = LaunchBuild(M, L, device, params);
+    } catch (...) {
+      host.join();
+      throw;
+    }
+    host.join();
+    if (host_rc != PG_OK) throw std::runtime_error(pg_host_last_error());
+    Index idx;
+    idx.off.resize(size_t(n) + 1);
+    idx.ids.resize(size_t(n) * pg_host_final_width(&h));
+    const uint32_t proj_deg = uint32_t(r.projection.size() / n);
+    const uint32_t m_link = uint32_t(r.link.size() / n);
+    if (pg_host_entry_point(&h, gt_ids_.data(), &idx.ep) != PG_OK ||
+        pg_host_fusion_final(&h, base, bi_off.data(), bi_ids.data(),
+                             r.link.data(), m_link, r.projection.data(),
+                             proj_deg, idx.off.data(), idx.ids.data()) != PG_OK)
+      throw std::runtime_error(pg_host_last_error());
+    idx.ids.resize(idx.off[n]);
+    return idx;
+  }
+
+  // SaveGraph layout (tests/test_t2iend.cu:615-633).
+  static auto SaveIndex(Index const& idx, std::string const& path) -> void {
+    if (pg_host_save_index(path.c_str(), idx.ep, uint32_t(idx.off.size() - 1),
+                           idx.off.data(), idx.ids.data()) != PG_OK)
+      throw std::runtime_error(pg_host_last_error());
   }
 
  private:

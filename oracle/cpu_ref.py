@@ -75,6 +75,20 @@ def _declare(L):
     L.orc_gpu_stage.restype = ctypes.c_int
     L.orc_max_threads.restype = ctypes.c_int
     L.orc_set_threads.argtypes = [ctypes.c_int]
+    # host stages (oracle/cpu_fusion.cpp)
+    L.orc_cpu_compare.restype = ctypes.c_float
+    L.orc_cpu_compare.argtypes = [_P, _P, _U32]
+    L.orc_rng_prune.restype = _U32
+    L.orc_rng_prune.argtypes = [_P, _U32, _U32, _P, _P, _U32, _U32, _U32,
+                                ctypes.c_int, _P]
+    L.orc_entry_point.restype = _U32
+    L.orc_entry_point.argtypes = [_P, _U32, _U32, _U32]
+    L.orc_match_sup.argtypes = [_P, _U32, _U32, _P, _U32, _U32, _U32, _U32,
+                                _P, _P]
+    L.orc_bipartite.argtypes = [_P, _U32, _U32, _P, _U32, _U32, _U32, _U32,
+                                _P, _P]
+    L.orc_fusion_final.argtypes = [_P, _U32, _U32, _P, _P, _P, _U32, _P, _U32,
+                                   _U32, _P, _P]
 
 
 def _f32(a):
@@ -206,3 +220,66 @@ def gpu_stage(params, base, gt, stats=None):
     if rc != 0:
         raise RuntimeError(f"orc_gpu_stage failed: {rc}")
     return proj, link
+
+
+# ------------------------------------------------------------ host stages
+# Restatements of the reference's CPU routines around the GPU stage
+# (oracle/cpu_fusion.cpp).  Lists are returned as CSR (off, ids).
+
+def cpu_compare(x, y):
+    """The reference CPU distance `compare` (tests/test_t2iend.cu:31-78)."""
+    x, y = _f32(x), _f32(y)
+    return float(lib().orc_cpu_compare(_ptr(x), _ptr(y), x.shape[0]))
+
+
+def rng_prune(base, ids, dists, target, M, strict):
+    """RNGPrune (tests/test_t2iend.cu:99-177) of candidates (ids, dists),
+    sorted here by (distance, id)."""
+    base = _f32(base)
+    ids, dists = _u32(ids), _f32(dists)
+    out = np.empty(max(int(M), 1), np.uint32)
+    n = lib().orc_rng_prune(_ptr(base), base.shape[0], base.shape[1],
+                            _ptr(ids), _ptr(dists), ids.shape[0], target, M,
+                            1 if strict else 0, _ptr(out))
+    return out[:n].copy()
+
+
+def entry_point(gt, n_base):
+    gt = _u32(gt)
+    return int(lib().orc_entry_point(_ptr(gt), gt.shape[0], gt.shape[1],
+                                     n_base))
+
+
+def match_sup(base, gt, topn, M_nn):
+    base, gt = _f32(base), _u32(gt)
+    n = base.shape[0]
+    off = np.empty(n + 1, np.uint32)
+    ids = np.empty(max(n * M_nn, 1), np.uint32)
+    lib().orc_match_sup(_ptr(base), n, base.shape[1], _ptr(gt), gt.shape[0],
+                        gt.shape[1], topn, M_nn, _ptr(off), _ptr(ids))
+    return off, ids[:off[-1]].copy()
+
+
+def bipartite(base, gt, M_nn, rounds):
+    base, gt = _f32(base), _u32(gt)
+    n = base.shape[0]
+    off = np.empty(n + 1, np.uint32)
+    ids = np.empty(max(n * 2 * M_nn, 1), np.uint32)
+    lib().orc_bipartite(_ptr(base), n, base.shape[1], _ptr(gt), gt.shape[0],
+                        gt.shape[1], M_nn, rounds, _ptr(off), _ptr(ids))
+    return off, ids[:off[-1]].copy()
+
+
+def fusion_final(base, bi_off, bi_ids, link, proj, M_final):
+    base = _f32(base)
+    bi_off, bi_ids = _u32(bi_off), _u32(bi_ids)
+    link, proj = _u32(link), _u32(proj)
+    n = base.shape[0]
+    width = int(np.max(np.diff(bi_off.astype(np.int64)), initial=0)) + M_final
+    off = np.empty(n + 1, np.uint32)
+    ids = np.empty(max(n * width, 1), np.uint32)
+    lib().orc_fusion_final(_ptr(base), n, base.shape[1], _ptr(bi_off),
+                           _ptr(bi_ids), _ptr(link), link.shape[1],
+                           _ptr(proj), proj.shape[1], M_final, _ptr(off),
+                           _ptr(ids))
+    return off, ids[:off[-1]].copy()

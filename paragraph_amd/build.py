@@ -1,4 +1,5 @@
-"""Compiles the HIP sources into paragraph_amd/libparagraph_hip.so (gfx950).
+"""Compiles the HIP sources into paragraph_amd/libparagraph_hip.so (gfx950)
+and the host stages into paragraph_amd/libparagraph_host.so.
 
 Run as `python -m paragraph_amd.build`.  hipcc cross-compiles without a GPU,
 so this works on a CPU-only machine; the shared library then travels with the
@@ -55,7 +56,31 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+HOST_LIB = os.path.join(HERE, "libparagraph_host.so")
+HOST_SOURCES = ["pg_host_stage.cpp"]
+
+
+def build_host(verbose=False):
+    """The host stages (entry point, supply / bipartite graphs, final lists,
+    index file) into libparagraph_host.so: plain C++ with OpenMP, explicit
+    FMAs (no contraction)."""
+    cxx = shutil.which("g++") or "g++"
+    tmp = HOST_LIB + ".tmp"
+    cmd = [cxx, "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared",
+           "-ffp-contract=off", "-mavx2", "-mfma", "-Wall", "-Wextra",
+           *[os.path.join(CSRC, s) for s in HOST_SOURCES], "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("host stage build failed")
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
 
 
 if __name__ == "__main__":

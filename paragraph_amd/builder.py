@@ -8,10 +8,17 @@ Same constructor arguments and SetGt; LaunchBuild takes the build knobs at
 run time (the reference bakes them into template arguments,
 tests/test_t2iend.cu:670-727) and returns the two arrays the reference copies
 back to the host: the projection rows (N x 16) and the link rows (N x M).
+
+BuildIndex() is the reference's whole build (tests/test_t2iend.cu:850-1258):
+the GPU stage, with the host bipartite stage (MatchSup rounds + FusionNN)
+running on a CPU thread beside it, then FusionFinal, the entry point and,
+given a path, the index file in the SaveGraph layout.
 """
+import threading
+
 import numpy as np
 
-from . import _native
+from . import _native, host_stage
 
 
 class Builder:
@@ -52,3 +59,37 @@ class Builder:
             ctx.wait()
             self.stats_ = ctx.stats().as_dict()
         return proj, link
+
+    def BuildIndex(self, M=32, L=128, path=None, device=0,
+                   precision=_native.PG_FP32, host_params=None, **overrides):
+        """Returns (ep, off, ids): the final lists as CSR.  host_params:
+        host_stage.make_params overrides (M_nn, rounds, M_final, threads)."""
+        n, dim = self.base_.shape
+        hp = host_stage.make_params(n, dim, self.gt_ids_.shape[0],
+                                    gt_k=self.gt_ids_.shape[1],
+                                    **(host_params or {}))
+        bip = {}
+
+        def host_work():
+            try:
+                bip["csr"] = host_stage.bipartite(hp, self.base_, self.gt_ids_)
+            except Exception as e:  # re-raised on the caller's thread
+                bip["err"] = e
+
+        # the bipartite stage overlaps the GPU stage, as in the reference
+        # (a CPU thread beside compute_stream, tests/test_t2iend.cu:888-962)
+        t = threading.Thread(target=host_work)
+        t.start()
+        try:
+            proj, link = self.LaunchBuild(M=M, L=L, device=device,
+                                          precision=precision, **overrides)
+        finally:
+            t.join()
+        if "err" in bip:
+            raise bip["err"]
+        ep = host_stage.entry_point(hp, self.gt_ids_)
+        off, ids = host_stage.fusion_final(hp, self.base_, *bip["csr"], link,
+                                           proj)
+        if path is not None:
+            host_stage.save_index(path, ep, off, ids)
+        return ep, off, ids

@@ -36,3 +36,26 @@ def test_builder_screened_by_name(cuda):
                            max(1, (c != golden_io.TOMB).sum())
                            for a, c in zip(link, g["link"])])
         assert overlap > 0.8, (prec, overlap)
+
+
+def test_build_index_matches_oracle(cuda, tmp_path):
+    """The whole build (GPU stage with the host bipartite stage beside it,
+    FusionFinal, entry point, index file) equals the oracle's composition of
+    the same stages, id for id."""
+    from oracle import cpu_ref
+    from paragraph_amd import io
+    g = golden_io.load_stage("stage_d200")
+    pr = g["params"]
+    b = Builder(g["base"], g["gt"].view(np.int32))
+    path = str(tmp_path / "index.bin")
+    ep, off, ids = b.BuildIndex(M=pr["M"], L=pr["Km"], path=path)
+    gt = np.ascontiguousarray(g["gt"], dtype=np.uint32)
+    bo, bi = cpu_ref.bipartite(g["base"], gt, 40, 6)
+    fo, fi = cpu_ref.fusion_final(g["base"], bo, bi, g["link"], g["proj"], 70)
+    assert ep == cpu_ref.entry_point(gt, pr["n_base"])
+    np.testing.assert_array_equal(off, fo)
+    np.testing.assert_array_equal(ids, fi)
+    ep_f, lists = io.load_index(path)
+    assert ep_f == ep
+    assert all(np.array_equal(lists[i], fi[fo[i]:fo[i + 1]])
+               for i in range(pr["n_base"]))
