@@ -93,7 +93,8 @@ typedef struct pg_params {
                            kept-major sweeps (same rows); bit 9 (0x200):
                            the persistent search takes its queries in a
                            locality order (nodes grouped by nearest pivot
-                           row, one queue segment per XCD; same rows) */
+                           row, one queue segment per XCD), and the
+                           link-row prune sweep its nodes (same rows) */
   uint32_t reserved[7];
 } pg_params;
 

@@ -144,10 +144,13 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     pc = torch.empty(n, **i32)
     splits = (0, 0x80) if p.precision != _native.PG_FP32 else (0,)
     best_pf, best_pt = 0, float("inf")
+    grid = best["tune_flags"] & 0x220 if best is not None else 0x20
     for split in splits:
         for scan in (0, 0x100):
             pp = _native.PgParams.from_buffer_copy(p)
-            pp.tune_flags = (p.tune_flags & ~0x180) | split | scan
+            # the chosen grid shape: with the locality order the sweep also
+            # takes its nodes in that order
+            pp.tune_flags = ((p.tune_flags & ~0x3A0) | grid | split | scan)
             with _native.Context(pp) as ctx:
                 t = _time(lambda: (ctx.run_prune_rows(base_d, full, p.M, pr,
                                                       pc), ctx.sync()))

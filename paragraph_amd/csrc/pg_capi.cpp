@@ -144,7 +144,7 @@ uint32_t effective_table_size(const pg_params& p) {
 //             order) instead of the kept-major sweep
 //   bit 9     persistent search takes its queries in a locality order
 //             (nodes grouped by nearest pivot row, one queue segment per
-//             XCD); same rows
+//             XCD), and the link-row prune sweep its nodes; same rows
 constexpr uint32_t kTunePersistent = 0x20u;
 constexpr uint32_t kTuneGraph = 0x40u;
 constexpr uint32_t kTuneSplit = 0x80u;
@@ -258,7 +258,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_prune(d_base, N, p.dim, nullptr, p.gt_k, d_gt,
                              c->winner, p.gt_k, p.proj_pr, c->pruned, c->pcnt,
-                             c->stats, nullptr, false, s));
+                             c->stats, nullptr, false, nullptr, s));
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.proj_pr, c->deg,
                                c->off, c->cursor, c->scan_tmp, c->srcs, s));
@@ -290,7 +290,8 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, width, nullptr,
                                nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
-                               scp, !(p.tune_flags & kTuneScanPrune), s));
+                               scp, !(p.tune_flags & kTuneScanPrune), sorder,
+                               s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.M, c->deg, c->off,
                                  c->cursor, c->scan_tmp, c->srcs, s));
@@ -707,10 +708,14 @@ int pg_run_prune(pg_ctx* c, const float* d_base, const uint32_t* d_rows,
   PG_HIP(c, hipSetDevice(c->device));
   const pg::ScreenArgs sc = screen_args(c);
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
+  // link rows take nodes in the locality order when the context has one
+  const uint32_t* sorder = nullptr;
+  if (d_rows) PG_TRY(search_order(c, d_base, c->p.n_base, &sorder));
   PG_HIP(c, pg::launch_prune(d_base, c->p.n_base, c->p.dim, d_rows, D, d_gt,
                              d_winner, gt_k, mp, d_pruned, d_pcnt, c->stats,
                              c->base16 ? &sc : nullptr,
-                             !(c->p.tune_flags & kTuneScanPrune), c->stream));
+                             !(c->p.tune_flags & kTuneScanPrune), sorder,
+                             c->stream));
   return PG_OK;
 }
 

@@ -37,7 +37,7 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
                         const uint32_t* winner, uint32_t gt_k, uint32_t mp,
                         uint32_t* pruned, uint32_t* pcnt,
                         unsigned long long* stats, const ScreenArgs* sc,
-                        bool sweep, hipStream_t s);
+                        bool sweep, const uint32_t* order, hipStream_t s);
 hipError_t launch_reverse(const uint32_t* pruned, const uint32_t* pcnt,
                           uint32_t n_base, uint32_t mp, uint32_t* deg,
                           uint32_t* off, uint32_t* cursor, uint32_t* scan_tmp,

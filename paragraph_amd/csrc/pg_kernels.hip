@@ -918,11 +918,14 @@ __global__ __launch_bounds__(64) void prune_sweep_kernel(
     const float* __restrict__ base, uint32_t n_base, uint32_t dim,
     const uint32_t* __restrict__ rows, uint32_t D, uint32_t mp,
     uint32_t* __restrict__ pruned, uint32_t* __restrict__ pcnt,
-    unsigned long long* __restrict__ stats, ScreenArgs sc) {
+    unsigned long long* __restrict__ stats, ScreenArgs sc,
+    const uint32_t* __restrict__ order) {
   __shared__ SweepLds<DP2, NCH, kScreen> L;
   // row ids, read before the first sweep writes L.sv
   uint32_t* sid = reinterpret_cast<uint32_t*>(L.sv);
-  const uint32_t b = blockIdx.x;
+  // nodes in the locality order when one is given (rows are per node, so
+  // the launch order changes no result)
+  const uint32_t b = order ? order[blockIdx.x] : blockIdx.x;
   const uint32_t lane = lane_id();
   // Entries: valid candidates, and self entries before the first stop id
   // (prune_kernel sorts them at FLT_MAX by position).
@@ -2034,7 +2037,7 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
                         const uint32_t* winner, uint32_t gt_k, uint32_t mp,
                         uint32_t* pruned, uint32_t* pcnt,
                         unsigned long long* stats, const ScreenArgs* sc,
-                        bool sweep, hipStream_t s) {
+                        bool sweep, const uint32_t* order, hipStream_t s) {
   if (n_base == 0) return hipSuccess;
   const ScreenArgs scv =
       sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, nullptr, 0, 0.0f, 0.0f, 0.0f};
@@ -2045,7 +2048,7 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
     // cost as many bytes as the FP32 row)
 #define PG_SWEEP(DP, NC, SCR)                                        \
   prune_sweep_kernel<DP, NC, SCR><<<n_base, 64, 0, s>>>(              \
-      base, n_base, dim, rows, D, mp, pruned, pcnt, stats, scv)
+      base, n_base, dim, rows, D, mp, pruned, pcnt, stats, scv, order)
     if (sc == nullptr || sc->base16lo != nullptr) {
       if (D <= 256) PG_SWEEP(256, 0, false);
       else PG_SWEEP(512, 0, false);

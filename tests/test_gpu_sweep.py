@@ -20,6 +20,7 @@ pytestmark = pytest.mark.gpu
 TOMB = 0xFFFFFFFF
 SCAN = 0x100
 SPLIT = 0x80
+ORDER = 0x200  # locality order: the sweep takes its nodes in that order
 
 
 def _dev(a, torch):
@@ -91,8 +92,9 @@ CASES = [
 
 @pytest.mark.parametrize("prec,flags", [
     (_native.PG_FP32, 0x20), (_native.PG_BF16, 0x20),
-    (_native.PG_BF16, 0x20 | SPLIT), (_native.PG_FP8E4M3, 0x20)],
-    ids=["fp32", "bf16", "bf16-split", "fp8"])
+    (_native.PG_BF16, 0x20 | SPLIT), (_native.PG_FP8E4M3, 0x20),
+    (_native.PG_FP32, 0x20 | ORDER), (_native.PG_BF16, 0x20 | ORDER)],
+    ids=["fp32", "bf16", "bf16-split", "fp8", "fp32-order", "bf16-order"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "-".join(map(str, c)))
 def test_sweep_equals_scan_and_oracle(case, prec, flags, cuda):
     import torch
