@@ -26,6 +26,8 @@
 
 #include <stdint.h>
 
+#include "paragraph_host.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -211,6 +213,24 @@ int pg_run_pair_distances(pg_ctx* ctx, const float* d_queries,
                           uint32_t n_query, const float* d_base,
                           const uint32_t* d_ids, uint32_t k, int kind,
                           float* d_out);
+
+/* The host stages of the full build on the GPU: MatchSup rounds
+ * 1..hp->rounds, FusionNN and FusionFinal (tests/test_t2iend.cu:365-613),
+ * giving the lists of pg_host_bipartite + pg_host_fusion_final
+ * (paragraph_host.h) id for id, under the same rules C6-C8.  Device
+ * buffers: base (N x dim), gt (N x gt_k; needs n_query == n_base), proj
+ * (N x proj_deg) and link (N x M_link) -- the GPU stage's outputs; out:
+ * off (N + 1) and ids (N x pg_host_final_width(hp)), CSR.  hp->n_base and
+ * dim must be the context's.  Limits: gt_k <= 256, M_nn <= 64,
+ * rounds * M_nn <= 256, 2 M_nn + M_link + proj_deg <= 256, M_final <= 128.
+ * The entry point stays on the host (pg_host_entry_point: the reference's
+ * scan is sequential).  The workspace (about (256 + 5 M_nn + rounds *
+ * (M_nn + 1) + M_final) x 4 bytes per base) is allocated on the first call
+ * and kept by the context. */
+int pg_run_fusion(pg_ctx* ctx, const pg_host_params* hp, const float* d_base,
+                  const uint32_t* d_gt, const uint32_t* d_proj,
+                  uint32_t proj_deg, const uint32_t* d_link, uint32_t M_link,
+                  uint32_t* d_off, uint32_t* d_ids);
 
 int pg_sync(pg_ctx* ctx);  /* hipStreamSynchronize + error check */
 

@@ -49,7 +49,7 @@ EXPORTED = (
     "pg_gpu_stage_async", "pg_gpu_stage_device_async", "pg_wait", "pg_query",
     "pg_get_stats", "pg_reset_stats", "pg_get_stage_counters", "pg_last_error", "pg_stage_name",
     "pg_run_match", "pg_run_prune", "pg_run_reverse", "pg_run_merge",
-    "pg_run_search", "pg_run_pair_distances", "pg_sync",
+    "pg_run_search", "pg_run_pair_distances", "pg_sync", "pg_run_fusion",
 )
 
 _lib = None
@@ -99,6 +99,7 @@ def _declare(L):
     L.pg_run_merge.argtypes = [_P, _P, _P, _P, _U32, _P, _P, _U32, _U32,
                                ctypes.c_int, _P]
     L.pg_run_search.argtypes = [_P, _P, _U32, _P, _P, _U32, _P]
+    L.pg_run_fusion.argtypes = [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P]
     L.pg_run_pair_distances.argtypes = [_P, _P, _U32, _P, _P, _U32,
                                         ctypes.c_int, _P]
 
@@ -245,6 +246,15 @@ class Context:
             self._ctx, _dptr(queries), queries.shape[0], _dptr(base),
             _dptr(ids), ids.shape[1], 1 if chain else 0, _dptr(out)),
             self._ctx)
+
+    def run_fusion(self, hp, base, gt, proj, link, off, ids):
+        """The host stages on the GPU (pg_run_fusion): hp is a
+        host_stage.HostParams; off (N + 1) and ids (N x final width) are
+        device tensors."""
+        _check(lib().pg_run_fusion(
+            self._ctx, ctypes.byref(hp), _dptr(base), _dptr(gt), _dptr(proj),
+            proj.shape[1], _dptr(link), link.shape[1], _dptr(off),
+            _dptr(ids)), self._ctx)
 
     def sync(self):
         _check(lib().pg_sync(self._ctx), self._ctx)

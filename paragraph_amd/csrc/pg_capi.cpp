@@ -65,6 +65,8 @@ struct pg_ctx {
   uint32_t* order = nullptr;   // locality order of the search queries
   uint32_t* ogroup = nullptr;  // its pivot group per node
   uint32_t* ocnt = nullptr;    // its group counters
+  uint32_t* fuse_ws = nullptr;  // host stages on the GPU (pg_run_fusion)
+  uint64_t fuse_words = 0;
   bool capturing = false;                   // recording a hipGraph
   hipGraph_t graph = nullptr;               // captured pipeline
   hipGraphExec_t graph_exec = nullptr;
@@ -556,6 +558,7 @@ void pg_destroy(pg_ctx* c) {
   dev_free(c->order);
   dev_free(c->ogroup);
   dev_free(c->ocnt);
+  dev_free(c->fuse_ws);
   dev_free(c->h_base_dev);
   dev_free(c->h_gt_dev);
   dev_free(c->h_proj_dev);
@@ -790,6 +793,47 @@ int pg_run_pair_distances(pg_ctx* c, const float* d_queries,
                                  c->base8 ? nullptr : c->base16, c->base8,
                                  c->scale8, c->dpad, p.n_base, p.dim, d_ids,
                                  k, kind, d_out, c->stream));
+  return PG_OK;
+}
+
+int pg_run_fusion(pg_ctx* c, const pg_host_params* hp, const float* d_base,
+                  const uint32_t* d_gt, const uint32_t* d_proj,
+                  uint32_t proj_deg, const uint32_t* d_link, uint32_t M_link,
+                  uint32_t* d_off, uint32_t* d_ids) {
+  if (!c || !hp || !d_base || !d_gt || !d_off || !d_ids ||
+      (M_link && !d_link) || (proj_deg && !d_proj))
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  const pg_params& p = c->p;
+  if (hp->n_base != p.n_base || hp->dim != p.dim)
+    return set_err(c, PG_ERR_INVALID,
+                   "host params must have the context's n_base and dim");
+  if (hp->n_query != hp->n_base)
+    return set_err(c, PG_ERR_INVALID, "the GPU host stages need n_query == n_base");
+  if (hp->gt_k == 0 || hp->gt_k > 256)
+    return set_err(c, PG_ERR_INVALID, "gt_k must be in [1, 256]");
+  if (hp->M_nn == 0 || hp->M_nn > 64)
+    return set_err(c, PG_ERR_INVALID, "M_nn must be in [1, 64]");
+  if (hp->rounds >= hp->gt_k || uint64_t(hp->rounds) * hp->M_nn > 256)
+    return set_err(c, PG_ERR_INVALID,
+                   "rounds must be < gt_k and rounds * M_nn <= 256");
+  if (hp->M_final == 0 || hp->M_final > 128)
+    return set_err(c, PG_ERR_INVALID, "M_final must be in [1, 128]");
+  if (uint64_t(2) * hp->M_nn + M_link + proj_deg > 256)
+    return set_err(c, PG_ERR_INVALID,
+                   "2 * M_nn + M_link + proj_deg must be <= 256");
+  const pg::FusionShape f{hp->n_base, hp->dim, hp->gt_k, hp->M_nn,
+                          hp->rounds, hp->M_final, M_link, proj_deg};
+  PG_HIP(c, hipSetDevice(c->device));
+  const uint64_t words = pg::fusion_ws_words(f);
+  if (c->fuse_words < words) {
+    PG_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(c->fuse_ws);
+    c->fuse_words = 0;
+    PG_TRY(dev_alloc(c, &c->fuse_ws, words, "fusion workspace"));
+    c->fuse_words = words;
+  }
+  PG_HIP(c, pg::launch_fusion(f, d_base, d_gt, d_proj, d_link, c->fuse_ws,
+                              d_off, d_ids, c->stream));
   return PG_OK;
 }
 

@@ -84,6 +84,20 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          unsigned long long* stats, uint32_t* queue,
                          const uint32_t* order, hipStream_t s);
 
+// Host stages of the full build on the GPU (MatchSup rounds 1..rounds,
+// FusionNN, FusionFinal; the lists of libparagraph_host).  Needs
+// n_query == n_base, gt_k <= 256, rounds * M_nn <= 256,
+// 2 * M_nn + M_link + proj_deg <= 256, M_final <= 128 (checked by the
+// caller).  off_out: n_base + 1; ids_out: n_base * (2 M_nn + M_final).
+struct FusionShape {
+  uint32_t n_base, dim, gt_k, M_nn, rounds, M_final, M_link, proj_deg;
+};
+uint64_t fusion_ws_words(const FusionShape& f);
+hipError_t launch_fusion(const FusionShape& f, const float* base,
+                         const uint32_t* gt, const uint32_t* proj,
+                         const uint32_t* link, uint32_t* ws, uint32_t* off_out,
+                         uint32_t* ids_out, hipStream_t s);
+
 // Pair distances: chain != 0 gives the FP32 prune chain; else the search
 // evaluator selected like launch_search (k <= 1024).
 hipError_t launch_pair_dist(const float* queries, uint32_t n_query,

@@ -2721,4 +2721,479 @@ hipError_t launch_pair_dist(const float* queries, uint32_t n_query,
   return hipErrorInvalidValue;
 }
 
+// ------------------------------------------------- host stages on the GPU
+// The reference's CPU routines around the GPU stage -- MatchSup, FusionNN
+// and FusionFinal (tests/test_t2iend.cu:365-613) -- as wave-per-list
+// kernels under the canonical rules of include/paragraph_host.h (C6-C8), so
+// that the lists equal libparagraph_host's id for id.
+
+// The reference CPU distance `compare` (tests/test_t2iend.cu:31-78): 16 FMA
+// lanes over blocks of 16, folded to 8 (+ one block of 8) and 4 (+ one
+// block of 4), the masked remainder on lanes 1..r (the element past the row
+// end taken as 0, rule C8), then ((l0 + l1) + (l2 + l3)).  IEEE FP32 adds
+// and FMAs, so one thread's result is bit-equal to the host's.
+__device__ __forceinline__ float cpu_order_dist(const float* __restrict__ a,
+                                                const float* __restrict__ b,
+                                                uint32_t n) {
+  float s[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s[j] = 0.0f;
+  uint32_t i = 0;
+  for (; n - i >= 16; i += 16) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      s[j] = __builtin_fmaf(a[i + j], b[i + j], s[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s[j] = s[8 + j] + s[j];
+  if (n - i >= 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] = __builtin_fmaf(a[i + j], b[i + j], s[j]);
+    i += 8;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) s[j] = s[4 + j] + s[j];
+  if (n - i >= 4) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s[j] = __builtin_fmaf(a[i + j], b[i + j], s[j]);
+    i += 4;
+  }
+  const uint32_t r = n - i;
+  if (r > 0) {
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const bool on = j >= 1 && j <= r && i + j < n;
+      s[j] = __builtin_fmaf(on ? a[i + j] : 0.0f, on ? b[i + j] : 0.0f, s[j]);
+    }
+  }
+  return -((s[0] + s[1]) + (s[2] + s[3]));
+}
+
+constexpr uint32_t kFuseMaxKeep = 128;
+constexpr uint32_t kRngStrict = 0;    // RNGPrune, strict
+constexpr uint32_t kRngFill = 1;      // RNGPrune, non-strict fill
+constexpr uint32_t kRngSmallAsc = 2;  // <= M ids: ascending (C7); else strict
+
+// RNGPrune (tests/test_t2iend.cu:99-177) of one candidate list per wave.
+// The list's candidates are cand[l * cstride .. + ccnt[l]); ids >= N, the
+// node itself (tgt[l], or l) and repeated ids drop out (the callers' seen
+// sets), the rest is sorted by (distance, id).  Kept-major form: the first
+// remaining entry is kept and removes every later entry it occludes, which
+// keeps exactly the entries the reference's candidate loop keeps (an entry
+// survives iff no earlier kept entry occludes it; the cap stops both).  Its
+// second pass keeps nothing new.  The non-strict fill appends the other
+// entries in sorted order.  W: candidate slots (128 or 256).
+template <uint32_t W>
+__global__ __launch_bounds__(64) void fuse_rng_kernel(
+    const float* __restrict__ base, uint32_t n_base, uint32_t dim,
+    const uint32_t* __restrict__ cand, uint32_t cstride,
+    const uint32_t* __restrict__ ccnt, const uint32_t* __restrict__ tgt,
+    uint32_t M, uint32_t mode, uint32_t* __restrict__ out, uint32_t ostride,
+    uint32_t* __restrict__ ocnt) {
+  __shared__ uint32_t ids[W];
+  __shared__ uint64_t keys[W];
+  __shared__ float dv[W];
+  __shared__ uint8_t alive[W];
+  __shared__ uint8_t keptf[W];
+  __shared__ uint32_t kept[kFuseMaxKeep];
+  constexpr uint32_t S = W / kWave;
+  const uint32_t l = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t t = tgt ? tgt[l] : l;
+  uint32_t n = ccnt[l];
+  if (n > W) n = W;
+  if (t >= n_base) n = 0;  // a list without a node (an unmatched query)
+  for (uint32_t j = lane; j < W; j += kWave) {
+    uint32_t id = kTomb;
+    if (j < n) {
+      id = cand[size_t(l) * cstride + j];
+      if (id >= n_base || id == t) id = kTomb;
+    }
+    ids[j] = id;
+  }
+  __syncthreads();
+  sort_list(ids, W);
+  // distinct ids, ascending, compacted
+  uint32_t v[S];
+  bool keep[S];
+#pragma unroll
+  for (uint32_t sl = 0; sl < S; ++sl) {
+    const uint32_t j = lane + kWave * sl;
+    v[sl] = ids[j];
+    keep[sl] = v[sl] != kTomb && (j == 0 || ids[j - 1] != v[sl]);
+  }
+  __syncthreads();
+  uint32_t m = 0;
+#pragma unroll
+  for (uint32_t sl = 0; sl < S; ++sl) {
+    const uint64_t km = __ballot(keep[sl]);
+    if (keep[sl]) ids[m + __popcll(km & lanemask_lt())] = v[sl];
+    m += __popcll(km);
+  }
+  __syncthreads();
+  uint32_t* o = out + size_t(l) * ostride;
+  if (mode == kRngSmallAsc && m <= M) {
+    for (uint32_t j = lane; j < m; j += kWave) o[j] = ids[j];
+    if (lane == 0) ocnt[l] = m;
+    return;
+  }
+  const float* xt = base + size_t(t < n_base ? t : 0u) * dim;
+  for (uint32_t j = lane; j < W; j += kWave) {
+    uint64_t k = kPadKey;
+    if (j < m) {
+      const uint32_t id = ids[j];
+      const float d = cpu_order_dist(xt, base + size_t(id) * dim, dim);
+      k = (uint64_t(ordered_bits(d)) << 32) | id;
+    }
+    keys[j] = k;
+  }
+  __syncthreads();
+  sort_list(keys, W);
+  for (uint32_t j = lane; j < W; j += kWave) {
+    alive[j] = j < m ? 1 : 0;
+    keptf[j] = 0;
+    dv[j] = j < m ? from_ordered(uint32_t(keys[j] >> 32)) : FLT_MAX;
+  }
+  __syncthreads();
+  uint32_t nk = 0, pos = 0;
+  while (nk < M) {
+    uint32_t f = W;
+    for (uint32_t j0 = pos & ~(kWave - 1u); j0 < m && f == W; j0 += kWave) {
+      const uint32_t j = j0 + lane;
+      const uint64_t am = __ballot(j >= pos && j < m && alive[j]);
+      if (am) f = j0 + __ffsll((long long)am) - 1;
+    }
+    if (f == W) break;
+    const uint32_t kid = uint32_t(keys[f]);
+    if (lane == 0) {
+      kept[nk] = kid;
+      keptf[f] = 1;
+    }
+    ++nk;
+    pos = f + 1;
+    if (nk == M) break;
+    const float* xk = base + size_t(kid) * dim;
+    for (uint32_t j = pos + lane; j < m; j += kWave) {
+      if (alive[j] &&
+          cpu_order_dist(base + size_t(uint32_t(keys[j])) * dim, xk, dim) <
+              dv[j])
+        alive[j] = 0;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (mode == kRngFill && nk < M) {
+    for (uint32_t j0 = 0; j0 < m && nk < M; j0 += kWave) {
+      const uint32_t j = j0 + lane;
+      const bool add = j < m && !keptf[j];
+      const uint64_t am = __ballot(add);
+      const uint32_t r = __popcll(am & lanemask_lt());
+      if (add && nk + r < M) kept[nk + r] = uint32_t(keys[j]);
+      const uint32_t c = __popcll(am);
+      nk = nk + c < M ? nk + c : M;
+    }
+    __syncthreads();
+  }
+  for (uint32_t j = lane; j < nk; j += kWave) o[j] = kept[j];
+  if (lane == 0) ocnt[l] = nk;
+}
+
+// MatchSup's first-come match on gt column `col` (tests/test_t2iend.cu:
+// 376-390): the lowest query id wins a base.  owner must be all 1s before.
+__global__ void column_owner_kernel(const uint32_t* __restrict__ gt,
+                                    uint32_t Q, uint32_t K, uint32_t col,
+                                    uint32_t n_base,
+                                    uint32_t* __restrict__ owner) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Q) return;
+  const uint32_t b = gt[size_t(q) * K + col];
+  if (b < n_base) atomicMin(&owner[b], q);
+}
+
+// MatchSup forward lists (:405-428): list q belongs to the base it matched,
+// and its candidates are the gt row up to the first id >= N.
+__global__ void sup_forward_lists_kernel(const uint32_t* __restrict__ gt,
+                                         uint32_t Q, uint32_t K, uint32_t col,
+                                         uint32_t n_base,
+                                         const uint32_t* __restrict__ owner,
+                                         uint32_t* __restrict__ ccnt,
+                                         uint32_t* __restrict__ tgt) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Q) return;
+  const uint32_t* g = gt + size_t(q) * K;
+  const uint32_t b = g[col];
+  uint32_t c = 0;
+  if (b < n_base && owner[b] == q)
+    while (c < K && g[c] < n_base) ++c;
+  ccnt[q] = c;
+  tgt[q] = c ? b : kTomb;
+}
+
+// MatchSup merge candidates (:450-461), one wave per base b: the forward
+// list of b, then the sources of the cap lowest query ids among the forward
+// lists that keep b (rule C6; the reverse CSR's sources are query ids).
+__global__ __launch_bounds__(64) void sup_merge_lists_kernel(
+    const uint32_t* __restrict__ gt, uint32_t K, uint32_t col,
+    const uint32_t* __restrict__ owner, const uint32_t* __restrict__ fwd,
+    const uint32_t* __restrict__ fcnt, uint32_t M,
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ srcs,
+    uint32_t cap, uint32_t n_base, uint32_t* __restrict__ cand,
+    uint32_t cstride, uint32_t* __restrict__ ccnt) {
+  __shared__ uint32_t buf[2 * kSegChunk];
+  const uint32_t b = blockIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t s0 = off[b], seg = off[b + 1] - s0;
+  const uint32_t r = seg < cap ? seg : cap;
+  if (seg <= kSegChunk) {
+    uint32_t n2 = 64;
+    while (n2 < seg) n2 <<= 1;
+    for (uint32_t j = lane; j < n2; j += kWave)
+      buf[j] = j < seg ? srcs[s0 + j] : kTomb;
+    __syncthreads();
+    if (seg > 1) sort_list(buf, n2);
+  } else {
+    for (uint32_t j = lane; j < kSegChunk; j += kWave) buf[j] = srcs[s0 + j];
+    __syncthreads();
+    bitonic_sort(buf, kSegChunk);
+    for (uint32_t done = kSegChunk; done < seg; done += kSegChunk) {
+      for (uint32_t j = lane; j < kSegChunk; j += kWave) {
+        if (j >= cap) buf[j] = kTomb;
+        const uint32_t g = done + j;
+        buf[kSegChunk + j] = g < seg ? srcs[s0 + g] : kTomb;
+      }
+      __syncthreads();
+      bitonic_sort(buf, 2 * kSegChunk);
+    }
+  }
+  const uint32_t q0 = owner[b];
+  const uint32_t nf = q0 < 0xFFFFFFFFu ? fcnt[q0] : 0u;
+  uint32_t* c = cand + size_t(b) * cstride;
+  for (uint32_t j = lane; j < nf; j += kWave) c[j] = fwd[size_t(q0) * M + j];
+  for (uint32_t j = lane; j < r; j += kWave)
+    c[nf + j] = gt[size_t(buf[j]) * K + col];
+  if (lane == 0) ccnt[b] = nf + r;
+}
+
+// Concatenates per-node rows of up to `w` ids from `parts` row sets (row
+// set p at src + p * pstride, counts at cnt + p * n) into cand rows.
+__global__ void concat_rows_kernel(const uint32_t* __restrict__ src,
+                                   const uint32_t* __restrict__ cnt,
+                                   uint32_t parts, uint64_t pstride,
+                                   uint32_t w, uint32_t n,
+                                   uint32_t* __restrict__ cand,
+                                   uint32_t cstride,
+                                   uint32_t* __restrict__ ccnt) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  uint32_t k = 0;
+  uint32_t* c = cand + size_t(b) * cstride;
+  for (uint32_t p = 0; p < parts; ++p) {
+    const uint32_t nc = cnt[size_t(p) * n + b];
+    const uint32_t* row = src + p * pstride + size_t(b) * w;
+    for (uint32_t j = 0; j < nc; ++j) c[k++] = row[j];
+  }
+  ccnt[b] = k;
+}
+
+// FusionFinal candidates (:553-582): the bipartite list, the link row and
+// the projection row of each base (ids >= N, the node and repeats drop out
+// in fuse_rng_kernel; the node itself, which the reference admits from the
+// link row, is never kept and occludes nothing, so dropping it changes no
+// list).
+__global__ void final_lists_kernel(const uint32_t* __restrict__ bip,
+                                   const uint32_t* __restrict__ bcnt,
+                                   uint32_t wb,
+                                   const uint32_t* __restrict__ link,
+                                   uint32_t M_link,
+                                   const uint32_t* __restrict__ proj,
+                                   uint32_t proj_deg, uint32_t n,
+                                   uint32_t* __restrict__ cand,
+                                   uint32_t cstride,
+                                   uint32_t* __restrict__ ccnt) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  uint32_t* c = cand + size_t(b) * cstride;
+  uint32_t k = 0;
+  for (uint32_t j = 0; j < bcnt[b]; ++j) c[k++] = bip[size_t(b) * wb + j];
+  for (uint32_t j = 0; j < M_link; ++j) c[k++] = link[size_t(b) * M_link + j];
+  for (uint32_t j = 0; j < proj_deg; ++j) c[k++] = proj[size_t(b) * proj_deg + j];
+  ccnt[b] = k;
+}
+
+// FusionFinal's output row (:553-603): the bipartite entries (first
+// occurrences, not the node, < N) in their order, then the pruned entries
+// not among them, while fewer than M_final - |bipartite| were added; that
+// room is unsigned in the reference, so a bipartite list longer than
+// M_final takes every pruned entry.
+__global__ void final_compose_kernel(const uint32_t* __restrict__ bip,
+                                     const uint32_t* __restrict__ bcnt,
+                                     uint32_t wb,
+                                     const uint32_t* __restrict__ pr,
+                                     const uint32_t* __restrict__ prc,
+                                     uint32_t M_final, uint32_t n_base,
+                                     uint32_t* __restrict__ rows,
+                                     uint32_t wr, uint32_t* __restrict__ rcnt) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n_base) return;
+  uint32_t* row = rows + size_t(b) * wr;
+  uint32_t nr = 0;
+  for (uint32_t j = 0; j < bcnt[b]; ++j) {
+    const uint32_t id = bip[size_t(b) * wb + j];
+    if (id >= n_base || id == b) continue;
+    bool dup = false;
+    for (uint32_t i = 0; i < nr && !dup; ++i) dup = row[i] == id;
+    if (!dup) row[nr++] = id;
+  }
+  const uint32_t nb = nr;
+  const uint64_t room =
+      nb <= M_final ? uint64_t(M_final - nb) : ~uint64_t(0);
+  uint64_t added = 0;
+  for (uint32_t j = 0; j < prc[b] && added < room; ++j) {
+    const uint32_t id = pr[size_t(b) * M_final + j];
+    bool in = false;
+    for (uint32_t i = 0; i < nb && !in; ++i) in = row[i] == id;
+    if (!in) {
+      row[nr++] = id;
+      ++added;
+    }
+  }
+  rcnt[b] = nr;
+}
+
+// Fixed-stride rows -> CSR ids (offsets from an exclusive scan of the counts).
+__global__ void rows_to_csr_kernel(const uint32_t* __restrict__ rows,
+                                   uint32_t wr,
+                                   const uint32_t* __restrict__ off,
+                                   uint32_t n, uint32_t* __restrict__ ids) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const uint32_t o = off[b], c = off[b + 1] - o;
+  for (uint32_t j = 0; j < c; ++j) ids[o + j] = rows[size_t(b) * wr + j];
+}
+
+static hipError_t launch_fuse_rng(const float* base, uint32_t n_base,
+                                  uint32_t dim, const uint32_t* cand,
+                                  uint32_t cstride, const uint32_t* ccnt,
+                                  const uint32_t* tgt, uint32_t nl,
+                                  uint32_t W, uint32_t M, uint32_t mode,
+                                  uint32_t* out, uint32_t ostride,
+                                  uint32_t* ocnt, hipStream_t s) {
+  if (nl == 0) return hipSuccess;
+  if (M == 0 || M > kFuseMaxKeep || M > ostride) return hipErrorInvalidValue;
+  if (W <= 128)
+    fuse_rng_kernel<128><<<nl, 64, 0, s>>>(base, n_base, dim, cand, cstride,
+                                           ccnt, tgt, M, mode, out, ostride,
+                                           ocnt);
+  else if (W <= 256)
+    fuse_rng_kernel<256><<<nl, 64, 0, s>>>(base, n_base, dim, cand, cstride,
+                                           ccnt, tgt, M, mode, out, ostride,
+                                           ocnt);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// Workspace of launch_fusion, carved in this order from one buffer.
+struct FusionWs {
+  uint32_t *owner, *ccnt, *tgt, *fcnt, *deg, *cursor, *off, *bcnt, *scan_tmp;
+  uint32_t *fwd, *srcs, *cand, *sup, *scnt, *bip, *pr, *prc, *rows, *rcnt;
+  uint64_t words;
+};
+
+static FusionWs fusion_carve(const FusionShape& f, uint32_t* base_ptr) {
+  const uint64_t N = f.n_base, M = f.M_nn, R = f.rounds;
+  const uint64_t nb = (N + kScanBlock - 1) / kScanBlock + 1;
+  FusionWs w{};
+  uint64_t at = 0;
+  auto take = [&](uint64_t words) {
+    uint32_t* q = base_ptr ? base_ptr + at : nullptr;
+    at += words;
+    return q;
+  };
+  w.owner = take(N);
+  w.ccnt = take(N);
+  w.tgt = take(N);
+  w.fcnt = take(N);
+  w.deg = take(N);
+  w.cursor = take(N);
+  w.off = take(N + 1);
+  w.bcnt = take(N);
+  w.scan_tmp = take(nb);
+  w.fwd = take(N * M);
+  w.srcs = take(N * M);
+  w.cand = take(N * 256);
+  w.sup = take(R * N * M);
+  w.scnt = take(R * N);
+  w.bip = take(N * 2 * M);
+  w.pr = take(N * f.M_final);
+  w.prc = take(N);
+  w.rows = take(N * (2 * M + f.M_final));
+  w.rcnt = take(N);
+  w.words = at;
+  return w;
+}
+
+uint64_t fusion_ws_words(const FusionShape& f) {
+  return fusion_carve(f, nullptr).words;
+}
+
+hipError_t launch_fusion(const FusionShape& f, const float* base,
+                         const uint32_t* gt, const uint32_t* proj,
+                         const uint32_t* link, uint32_t* ws, uint32_t* off_out,
+                         uint32_t* ids_out, hipStream_t s) {
+  const uint32_t N = f.n_base, K = f.gt_k, M = f.M_nn, R = f.rounds;
+  const uint32_t dim = f.dim, cap = 128u - M, wf = 2 * M + f.M_final;
+  const uint32_t nblk = (N + kScanBlock - 1) / kScanBlock;
+  if (N == 0) return hipSuccess;
+  const FusionWs w = fusion_carve(f, ws);
+  uint32_t *owner = w.owner, *ccnt = w.ccnt, *tgt = w.tgt, *fcnt = w.fcnt;
+  uint32_t *deg = w.deg, *cursor = w.cursor, *off = w.off, *bcnt = w.bcnt;
+  uint32_t *scan_tmp = w.scan_tmp, *fwd = w.fwd, *srcs = w.srcs;
+  uint32_t *cand = w.cand, *sup = w.sup, *scnt = w.scnt, *bip = w.bip;
+  uint32_t *pr = w.pr, *prc = w.prc, *rows = w.rows, *rcnt = w.rcnt;
+  const uint32_t tb = 256, tg = (N + tb - 1) / tb;
+  hipError_t e = hipSuccess;
+  for (uint32_t r = 0; r < R && e == hipSuccess; ++r) {
+    const uint32_t col = r + 1;
+    e = launch_fill_u32(owner, N, 0xFFFFFFFFu, s);
+    if (e != hipSuccess) break;
+    column_owner_kernel<<<tg, tb, 0, s>>>(gt, N, K, col, N, owner);
+    sup_forward_lists_kernel<<<tg, tb, 0, s>>>(gt, N, K, col, N, owner, ccnt,
+                                               tgt);
+    e = launch_fuse_rng(base, N, dim, gt, K, ccnt, tgt, N, K, M, kRngFill,
+                        fwd, M, fcnt, s);
+    if (e != hipSuccess) break;
+    // reverse lists keyed by base, sources = query ids (rows of fwd)
+    e = launch_reverse(fwd, fcnt, N, M, deg, off, cursor, scan_tmp, srcs, s);
+    if (e != hipSuccess) break;
+    sup_merge_lists_kernel<<<N, 64, 0, s>>>(gt, K, col, owner, fwd, fcnt, M,
+                                            off, srcs, cap, N, cand, 128,
+                                            ccnt);
+    e = launch_fuse_rng(base, N, dim, cand, 128, ccnt, nullptr, N, 128, M,
+                        kRngSmallAsc, sup + uint64_t(r) * N * M, M,
+                        scnt + uint64_t(r) * N, s);
+  }
+  if (e != hipSuccess) return e;
+  // FusionNN: union of the supply lists, strict to 2 M
+  concat_rows_kernel<<<tg, tb, 0, s>>>(sup, scnt, R, uint64_t(N) * M, M, N,
+                                       cand, 256, ccnt);
+  e = launch_fuse_rng(base, N, dim, cand, 256, ccnt, nullptr, N, R * M,
+                      2 * M, kRngStrict, bip, 2 * M, bcnt, s);
+  if (e != hipSuccess) return e;
+  // FusionFinal
+  final_lists_kernel<<<tg, tb, 0, s>>>(bip, bcnt, 2 * M, link, f.M_link, proj,
+                                       f.proj_deg, N, cand, 256, ccnt);
+  e = launch_fuse_rng(base, N, dim, cand, 256, ccnt, nullptr, N,
+                      2 * M + f.M_link + f.proj_deg, f.M_final, kRngStrict,
+                      pr, f.M_final, prc, s);
+  if (e != hipSuccess) return e;
+  final_compose_kernel<<<tg, tb, 0, s>>>(bip, bcnt, 2 * M, pr, prc, f.M_final,
+                                         N, rows, wf, rcnt);
+  scan_blocks_kernel<<<nblk, 256, 0, s>>>(rcnt, N, off_out, scan_tmp);
+  scan_sums_kernel<<<1, 1024, 0, s>>>(scan_tmp, nblk, off_out + N);
+  scan_add_kernel<<<tg, tb, 0, s>>>(off_out, N, scan_tmp);
+  rows_to_csr_kernel<<<tg, tb, 0, s>>>(rows, wf, off_out, N, ids_out);
+  return hipGetLastError();
+}
+
 }  // namespace pg
