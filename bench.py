@@ -519,8 +519,40 @@ def time_end_to_end(results, precs, w, gt_u32, base_d, gt_d, n, M, log):
          "host_threads": host_stage.default_threads(),
          "entry_point": ep, "final_degree_mean": round(float(lens.mean()), 2),
          "final_degree_max": int(lens.max())}
-    log(f"end-to-end build ({prec}): {r}")
-    return r
+    log(f"end-to-end build, host stages on the host ({prec}): {r}")
+    # the same build with the host stages on the GPU (pg_run_fusion, same
+    # stream after the GPU stage); only the sequential entry-point scan
+    # stays on the host, beside it
+    width = int(host_stage.lib().pg_host_final_width(
+        host_stage.ctypes.byref(hp)))
+    off_d = torch.empty(n + 1, dtype=torch.int32, device=base_d.device)
+    ids_d = torch.empty(n * width, dtype=torch.int32, device=base_d.device)
+    ctx.run_fusion(hp, base_d, gt_d, proj_d, link_d, off_d, ids_d)  # warm-up
+    ctx.sync()
+    gtimes = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        eps = {}
+        t0 = time.perf_counter()
+        th = threading.Thread(
+            target=lambda: eps.setdefault("ep", host_stage.entry_point(
+                hp, gt_u32)))
+        th.start()
+        ctx.stage_device(base_d, gt_d, proj_d, link_d)
+        ctx.run_fusion(hp, base_d, gt_d, proj_d, link_d, off_d, ids_d)
+        ctx.wait()
+        off_g = off_d.cpu().numpy().view(np.uint32)
+        ids_g = ids_d[:int(off_g[-1])].cpu().numpy().view(np.uint32)
+        th.join()
+        gtimes.append(time.perf_counter() - t0)
+    tg = min(gtimes)
+    r_gpu = {"precision": prec, "ms": round(1000 * tg, 1),
+             "base_points_per_s": round(n / tg, 1),
+             "same_lists_as_host_stages": bool(
+                 np.array_equal(off_g, off) and np.array_equal(ids_g, ids)
+                 and eps["ep"] == ep)}
+    log(f"end-to-end build, host stages on the GPU ({prec}): {r_gpu}")
+    return {"host_stages_on_host": r, "host_stages_on_gpu": r_gpu}
 
 
 def distance_error(ctxs, w, base_d, n_sample=2000):
