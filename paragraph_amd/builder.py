@@ -61,13 +61,23 @@ class Builder:
         return proj, link
 
     def BuildIndex(self, M=32, L=128, path=None, device=0,
-                   precision=_native.PG_FP32, host_params=None, **overrides):
+                   precision=_native.PG_FP32, host_params=None,
+                   host_stages="gpu", **overrides):
         """Returns (ep, off, ids): the final lists as CSR.  host_params:
-        host_stage.make_params overrides (M_nn, rounds, M_final, threads)."""
+        host_stage.make_params overrides (M_nn, rounds, M_final, threads).
+        host_stages="gpu" runs MatchSup / FusionNN / FusionFinal on the GPU
+        after the GPU stage (pg_run_fusion); "host" runs them on the host
+        cores, the bipartite part beside the GPU stage as in the reference.
+        Both give the same lists."""
         n, dim = self.base_.shape
         hp = host_stage.make_params(n, dim, self.gt_ids_.shape[0],
                                     gt_k=self.gt_ids_.shape[1],
                                     **(host_params or {}))
+        if host_stages == "gpu":
+            return self._build_index_gpu(hp, M, L, path, device, precision,
+                                         overrides)
+        if host_stages != "host":
+            raise ValueError('host_stages must be "gpu" or "host"')
         bip = {}
 
         def host_work():
@@ -90,6 +100,46 @@ class Builder:
         ep = host_stage.entry_point(hp, self.gt_ids_)
         off, ids = host_stage.fusion_final(hp, self.base_, *bip["csr"], link,
                                            proj)
+        if path is not None:
+            host_stage.save_index(path, ep, off, ids)
+        return ep, off, ids
+
+    def _build_index_gpu(self, hp, M, L, path, device, precision, overrides):
+        import torch
+        if isinstance(precision, str):
+            precision = {"fp32": _native.PG_FP32, "bf16": _native.PG_BF16,
+                         "fp8": _native.PG_FP8E4M3}[precision]
+        n, dim = self.base_.shape
+        q, gt_k = self.gt_ids_.shape
+        p = _native.make_params(n, dim, q, M, L, gt_k=gt_k,
+                                precision=precision, **overrides)
+        dev = torch.device(f"cuda:{device}")
+        base_d = torch.from_numpy(self.base_).to(dev)
+        gt_d = torch.from_numpy(self.gt_ids_.view(np.int32)).to(dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        proj_d = torch.empty((n, p.proj_deg), **i32)
+        link_d = torch.empty((n, p.M), **i32)
+        width = int(host_stage.lib().pg_host_final_width(
+            host_stage.ctypes.byref(hp)))
+        off_d = torch.empty(n + 1, **i32)
+        ids_d = torch.empty(max(n * width, 1), **i32)
+        eps = {}
+        # the entry-point scan (sequential) runs on the host meanwhile
+        t = threading.Thread(target=lambda: eps.setdefault(
+            "ep", host_stage.entry_point(hp, self.gt_ids_)))
+        t.start()
+        try:
+            with _native.Context(p, device) as ctx:
+                ctx.stage_device(base_d, gt_d, proj_d, link_d)
+                ctx.run_fusion(hp, base_d, gt_d, proj_d, link_d, off_d,
+                               ids_d)
+                ctx.wait()
+                self.stats_ = ctx.stats().as_dict()
+        finally:
+            t.join()
+        off = off_d.cpu().numpy().view(np.uint32)
+        ids = ids_d[:int(off[-1])].cpu().numpy().view(np.uint32).copy()
+        ep = eps["ep"]
         if path is not None:
             host_stage.save_index(path, ep, off, ids)
         return ep, off, ids

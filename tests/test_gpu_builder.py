@@ -38,7 +38,8 @@ def test_builder_screened_by_name(cuda):
         assert overlap > 0.8, (prec, overlap)
 
 
-def test_build_index_matches_oracle(cuda, tmp_path):
+@pytest.mark.parametrize("host_stages", ["gpu", "host"])
+def test_build_index_matches_oracle(host_stages, cuda, tmp_path):
     """The whole build (GPU stage with the host bipartite stage beside it,
     FusionFinal, entry point, index file) equals the oracle's composition of
     the same stages, id for id."""
@@ -48,7 +49,8 @@ def test_build_index_matches_oracle(cuda, tmp_path):
     pr = g["params"]
     b = Builder(g["base"], g["gt"].view(np.int32))
     path = str(tmp_path / "index.bin")
-    ep, off, ids = b.BuildIndex(M=pr["M"], L=pr["Km"], path=path)
+    ep, off, ids = b.BuildIndex(M=pr["M"], L=pr["Km"], path=path,
+                                host_stages=host_stages)
     gt = np.ascontiguousarray(g["gt"], dtype=np.uint32)
     bo, bi = cpu_ref.bipartite(g["base"], gt, 40, 6)
     fo, fi = cpu_ref.fusion_final(g["base"], bo, bi, g["link"], g["proj"], 70)
