@@ -520,9 +520,9 @@ def time_end_to_end(results, precs, w, gt_u32, base_d, gt_d, n, M, log):
          "entry_point": ep, "final_degree_mean": round(float(lens.mean()), 2),
          "final_degree_max": int(lens.max())}
     log(f"end-to-end build, host stages on the host ({prec}): {r}")
-    # the same build with the host stages on the GPU (pg_run_fusion, same
-    # stream after the GPU stage); only the sequential entry-point scan
-    # stays on the host, beside it
+    # the same build with the host stages on the GPU: the bipartite stage
+    # beside the GPU stage on a second stream, FusionFinal after it; only
+    # the sequential entry-point scan stays on the host, beside both
     width = int(host_stage.lib().pg_host_final_width(
         host_stage.ctypes.byref(hp)))
     off_d = torch.empty(n + 1, dtype=torch.int32, device=base_d.device)
@@ -538,8 +538,11 @@ def time_end_to_end(results, precs, w, gt_u32, base_d, gt_d, n, M, log):
             target=lambda: eps.setdefault("ep", host_stage.entry_point(
                 hp, gt_u32)))
         th.start()
+        # the bipartite stage on the context's second stream beside the GPU
+        # stage, then FusionFinal on the main stream
+        ctx.run_bipartite_async(hp, base_d, gt_d)
         ctx.stage_device(base_d, gt_d, proj_d, link_d)
-        ctx.run_fusion(hp, base_d, gt_d, proj_d, link_d, off_d, ids_d)
+        ctx.run_fusion_final(hp, base_d, proj_d, link_d, off_d, ids_d)
         ctx.wait()
         off_g = off_d.cpu().numpy().view(np.uint32)
         ids_g = ids_d[:int(off_g[-1])].cpu().numpy().view(np.uint32)

@@ -232,6 +232,21 @@ int pg_run_fusion(pg_ctx* ctx, const pg_host_params* hp, const float* d_base,
                   uint32_t proj_deg, const uint32_t* d_link, uint32_t M_link,
                   uint32_t* d_off, uint32_t* d_ids);
 
+/* The two halves of pg_run_fusion, so that the bipartite stage overlaps
+ * the GPU stage as the reference's CPU thread does (tests/test_t2iend.cu:
+ * 888-962): pg_run_bipartite_async enqueues MatchSup rounds + FusionNN
+ * (they read only base and gt) on the context's second stream, ordered
+ * after the work already on its main stream; pg_run_fusion_final enqueues
+ * FusionFinal on the main stream after both the bipartite stage and what
+ * the main stream holds (e.g. pg_gpu_stage_device_async).  hp must have
+ * the same shape in both calls.  pg_wait / pg_query cover both streams. */
+int pg_run_bipartite_async(pg_ctx* ctx, const pg_host_params* hp,
+                           const float* d_base, const uint32_t* d_gt);
+int pg_run_fusion_final(pg_ctx* ctx, const pg_host_params* hp,
+                        const float* d_base, const uint32_t* d_proj,
+                        uint32_t proj_deg, const uint32_t* d_link,
+                        uint32_t M_link, uint32_t* d_off, uint32_t* d_ids);
+
 int pg_sync(pg_ctx* ctx);  /* hipStreamSynchronize + error check */
 
 #ifdef __cplusplus

@@ -50,6 +50,7 @@ EXPORTED = (
     "pg_get_stats", "pg_reset_stats", "pg_get_stage_counters", "pg_last_error", "pg_stage_name",
     "pg_run_match", "pg_run_prune", "pg_run_reverse", "pg_run_merge",
     "pg_run_search", "pg_run_pair_distances", "pg_sync", "pg_run_fusion",
+    "pg_run_bipartite_async", "pg_run_fusion_final",
 )
 
 _lib = None
@@ -100,6 +101,8 @@ def _declare(L):
                                ctypes.c_int, _P]
     L.pg_run_search.argtypes = [_P, _P, _U32, _P, _P, _U32, _P]
     L.pg_run_fusion.argtypes = [_P, _P, _P, _P, _P, _U32, _P, _U32, _P, _P]
+    L.pg_run_bipartite_async.argtypes = [_P, _P, _P, _P]
+    L.pg_run_fusion_final.argtypes = [_P, _P, _P, _P, _U32, _P, _U32, _P, _P]
     L.pg_run_pair_distances.argtypes = [_P, _P, _U32, _P, _P, _U32,
                                         ctypes.c_int, _P]
 
@@ -253,6 +256,21 @@ class Context:
         device tensors."""
         _check(lib().pg_run_fusion(
             self._ctx, ctypes.byref(hp), _dptr(base), _dptr(gt), _dptr(proj),
+            proj.shape[1], _dptr(link), link.shape[1], _dptr(off),
+            _dptr(ids)), self._ctx)
+
+    def run_bipartite_async(self, hp, base, gt):
+        """MatchSup rounds + FusionNN on the context's second stream
+        (pg_run_bipartite_async)."""
+        _check(lib().pg_run_bipartite_async(self._ctx, ctypes.byref(hp),
+                                            _dptr(base), _dptr(gt)),
+               self._ctx)
+
+    def run_fusion_final(self, hp, base, proj, link, off, ids):
+        """FusionFinal after the bipartite stage and the main stream's work
+        (pg_run_fusion_final)."""
+        _check(lib().pg_run_fusion_final(
+            self._ctx, ctypes.byref(hp), _dptr(base), _dptr(proj),
             proj.shape[1], _dptr(link), link.shape[1], _dptr(off),
             _dptr(ids)), self._ctx)
 

@@ -67,6 +67,11 @@ struct pg_ctx {
   uint32_t* ocnt = nullptr;    // its group counters
   uint32_t* fuse_ws = nullptr;  // host stages on the GPU (pg_run_fusion)
   uint64_t fuse_words = 0;
+  hipStream_t aux = nullptr;    // second stream: the bipartite stage beside
+  hipEvent_t ev_in = nullptr;   // the GPU stage (pg_run_bipartite_async)
+  hipEvent_t ev_bip = nullptr;
+  bool bip_valid = false;       // the workspace holds bipartite rows for
+  pg_host_params bip_hp = {};   // this shape
   bool capturing = false;                   // recording a hipGraph
   hipGraph_t graph = nullptr;               // captured pipeline
   hipGraphExec_t graph_exec = nullptr;
@@ -535,6 +540,14 @@ void pg_destroy(pg_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->aux) {
+    (void)hipStreamSynchronize(c->aux);
+    (void)hipStreamDestroy(c->aux);
+    c->aux = nullptr;
+  }
+  if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+  if (c->ev_bip) (void)hipEventDestroy(c->ev_bip);
+  c->ev_in = c->ev_bip = nullptr;
   dev_free(c->winner);
   dev_free(c->pruned);
   dev_free(c->pcnt);
@@ -619,6 +632,7 @@ int pg_gpu_stage_async(pg_ctx* c, const float* base, const uint32_t* gt,
 int pg_wait(pg_ctx* c) {
   if (!c) return set_err(nullptr, PG_ERR_INVALID, "ctx is NULL");
   PG_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->aux) PG_HIP(c, hipStreamSynchronize(c->aux));
   PG_HIP(c, hipGetLastError());
   return PG_OK;
 }
@@ -628,6 +642,7 @@ int pg_sync(pg_ctx* c) { return pg_wait(c); }
 int pg_query(pg_ctx* c) {
   if (!c) return set_err(nullptr, PG_ERR_INVALID, "ctx is NULL");
   hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipSuccess && c->aux) e = hipStreamQuery(c->aux);
   if (e == hipSuccess) return PG_OK;
   if (e == hipErrorNotReady) return PG_ERR_BUSY;
   return set_err(c, PG_ERR_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
@@ -796,13 +811,11 @@ int pg_run_pair_distances(pg_ctx* c, const float* d_queries,
   return PG_OK;
 }
 
-int pg_run_fusion(pg_ctx* c, const pg_host_params* hp, const float* d_base,
-                  const uint32_t* d_gt, const uint32_t* d_proj,
-                  uint32_t proj_deg, const uint32_t* d_link, uint32_t M_link,
-                  uint32_t* d_off, uint32_t* d_ids) {
-  if (!c || !hp || !d_base || !d_gt || !d_off || !d_ids ||
-      (M_link && !d_link) || (proj_deg && !d_proj))
-    return set_err(c, PG_ERR_INVALID, "NULL argument");
+}  // extern "C"
+
+namespace {
+
+int check_fusion_shape(pg_ctx* c, const pg_host_params* hp) {
   const pg_params& p = c->p;
   if (hp->n_base != p.n_base || hp->dim != p.dim)
     return set_err(c, PG_ERR_INVALID,
@@ -818,22 +831,108 @@ int pg_run_fusion(pg_ctx* c, const pg_host_params* hp, const float* d_base,
                    "rounds must be < gt_k and rounds * M_nn <= 256");
   if (hp->M_final == 0 || hp->M_final > 128)
     return set_err(c, PG_ERR_INVALID, "M_final must be in [1, 128]");
+  return PG_OK;
+}
+
+pg::FusionShape fusion_shape(const pg_host_params* hp, uint32_t M_link,
+                             uint32_t proj_deg) {
+  return pg::FusionShape{hp->n_base, hp->dim, hp->gt_k, hp->M_nn,
+                         hp->rounds, hp->M_final, M_link, proj_deg};
+}
+
+// Workspace for shape f (kept across calls; grown after both streams are
+// idle).
+int fusion_workspace(pg_ctx* c, const pg::FusionShape& f) {
+  const uint64_t words = pg::fusion_ws_words(f);
+  if (c->fuse_words >= words) return PG_OK;
+  PG_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->aux) PG_HIP(c, hipStreamSynchronize(c->aux));
+  dev_free(c->fuse_ws);
+  c->fuse_words = 0;
+  c->bip_valid = false;
+  PG_TRY(dev_alloc(c, &c->fuse_ws, words, "fusion workspace"));
+  c->fuse_words = words;
+  return PG_OK;
+}
+
+bool same_bip_shape(const pg_host_params& a, const pg_host_params& b) {
+  return a.n_base == b.n_base && a.dim == b.dim && a.n_query == b.n_query &&
+         a.gt_k == b.gt_k && a.M_nn == b.M_nn && a.rounds == b.rounds &&
+         a.M_final == b.M_final;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_run_bipartite_async(pg_ctx* c, const pg_host_params* hp,
+                           const float* d_base, const uint32_t* d_gt) {
+  if (!c || !hp || !d_base || !d_gt)
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  PG_TRY(check_fusion_shape(c, hp));
+  PG_HIP(c, hipSetDevice(c->device));
+  if (!c->aux) {
+    PG_HIP(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+    PG_HIP(c, hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+    PG_HIP(c, hipEventCreateWithFlags(&c->ev_bip, hipEventDisableTiming));
+  }
+  // the workspace is sized for the widest FusionFinal candidate rows
+  const pg::FusionShape f = fusion_shape(hp, 0, 0);
+  PG_TRY(fusion_workspace(c, f));
+  // after everything already on the context's stream, on the second stream
+  PG_HIP(c, hipEventRecord(c->ev_in, c->stream));
+  PG_HIP(c, hipStreamWaitEvent(c->aux, c->ev_in, 0));
+  PG_HIP(c, pg::launch_fusion_bipartite(f, d_base, d_gt, c->fuse_ws, c->aux));
+  PG_HIP(c, hipEventRecord(c->ev_bip, c->aux));
+  c->bip_valid = true;
+  c->bip_hp = *hp;
+  return PG_OK;
+}
+
+int pg_run_fusion_final(pg_ctx* c, const pg_host_params* hp,
+                        const float* d_base, const uint32_t* d_proj,
+                        uint32_t proj_deg, const uint32_t* d_link,
+                        uint32_t M_link, uint32_t* d_off, uint32_t* d_ids) {
+  if (!c || !hp || !d_base || !d_off || !d_ids || (M_link && !d_link) ||
+      (proj_deg && !d_proj))
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  PG_TRY(check_fusion_shape(c, hp));
   if (uint64_t(2) * hp->M_nn + M_link + proj_deg > 256)
     return set_err(c, PG_ERR_INVALID,
                    "2 * M_nn + M_link + proj_deg must be <= 256");
-  const pg::FusionShape f{hp->n_base, hp->dim, hp->gt_k, hp->M_nn,
-                          hp->rounds, hp->M_final, M_link, proj_deg};
+  if (!c->bip_valid || !same_bip_shape(c->bip_hp, *hp))
+    return set_err(c, PG_ERR_INVALID,
+                   "no bipartite rows of this shape: call "
+                   "pg_run_bipartite_async (or pg_run_fusion) first");
   PG_HIP(c, hipSetDevice(c->device));
-  const uint64_t words = pg::fusion_ws_words(f);
-  if (c->fuse_words < words) {
-    PG_HIP(c, hipStreamSynchronize(c->stream));
-    dev_free(c->fuse_ws);
-    c->fuse_words = 0;
-    PG_TRY(dev_alloc(c, &c->fuse_ws, words, "fusion workspace"));
-    c->fuse_words = words;
-  }
+  if (c->aux) PG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_bip, 0));
+  PG_HIP(c, pg::launch_fusion_final(fusion_shape(hp, M_link, proj_deg),
+                                    d_base, d_proj, d_link, c->fuse_ws, d_off,
+                                    d_ids, c->stream));
+  return PG_OK;
+}
+
+int pg_run_fusion(pg_ctx* c, const pg_host_params* hp, const float* d_base,
+                  const uint32_t* d_gt, const uint32_t* d_proj,
+                  uint32_t proj_deg, const uint32_t* d_link, uint32_t M_link,
+                  uint32_t* d_off, uint32_t* d_ids) {
+  if (!c || !hp || !d_base || !d_gt || !d_off || !d_ids ||
+      (M_link && !d_link) || (proj_deg && !d_proj))
+    return set_err(c, PG_ERR_INVALID, "NULL argument");
+  PG_TRY(check_fusion_shape(c, hp));
+  if (uint64_t(2) * hp->M_nn + M_link + proj_deg > 256)
+    return set_err(c, PG_ERR_INVALID,
+                   "2 * M_nn + M_link + proj_deg must be <= 256");
+  PG_HIP(c, hipSetDevice(c->device));
+  const pg::FusionShape f = fusion_shape(hp, M_link, proj_deg);
+  PG_TRY(fusion_workspace(c, f));
+  // a bipartite stage still running on the second stream shares the
+  // workspace: order after it
+  if (c->aux) PG_HIP(c, hipStreamWaitEvent(c->stream, c->ev_bip, 0));
   PG_HIP(c, pg::launch_fusion(f, d_base, d_gt, d_proj, d_link, c->fuse_ws,
                               d_off, d_ids, c->stream));
+  c->bip_valid = true;
+  c->bip_hp = *hp;
   return PG_OK;
 }
 

@@ -97,6 +97,15 @@ hipError_t launch_fusion(const FusionShape& f, const float* base,
                          const uint32_t* gt, const uint32_t* proj,
                          const uint32_t* link, uint32_t* ws, uint32_t* off_out,
                          uint32_t* ids_out, hipStream_t s);
+// The two halves of launch_fusion: MatchSup + FusionNN (base and gt only)
+// into the workspace, then FusionFinal from it.
+hipError_t launch_fusion_bipartite(const FusionShape& f, const float* base,
+                                   const uint32_t* gt, uint32_t* ws,
+                                   hipStream_t s);
+hipError_t launch_fusion_final(const FusionShape& f, const float* base,
+                               const uint32_t* proj, const uint32_t* link,
+                               uint32_t* ws, uint32_t* off_out,
+                               uint32_t* ids_out, hipStream_t s);
 
 // Pair distances: chain != 0 gives the FP32 prune chain; else the search
 // evaluator selected like launch_search (k <= 1024).

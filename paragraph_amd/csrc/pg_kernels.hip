@@ -3137,21 +3137,29 @@ uint64_t fusion_ws_words(const FusionShape& f) {
   return fusion_carve(f, nullptr).words;
 }
 
-hipError_t launch_fusion(const FusionShape& f, const float* base,
-                         const uint32_t* gt, const uint32_t* proj,
-                         const uint32_t* link, uint32_t* ws, uint32_t* off_out,
-                         uint32_t* ids_out, hipStream_t s) {
-  const uint32_t N = f.n_base, K = f.gt_k, M = f.M_nn, R = f.rounds;
-  const uint32_t dim = f.dim, cap = 128u - M, wf = 2 * M + f.M_final;
-  const uint32_t nblk = (N + kScanBlock - 1) / kScanBlock;
-  if (N == 0) return hipSuccess;
-  const FusionWs w = fusion_carve(f, ws);
-  uint32_t *owner = w.owner, *ccnt = w.ccnt, *tgt = w.tgt, *fcnt = w.fcnt;
-  uint32_t *deg = w.deg, *cursor = w.cursor, *off = w.off, *bcnt = w.bcnt;
-  uint32_t *scan_tmp = w.scan_tmp, *fwd = w.fwd, *srcs = w.srcs;
-  uint32_t *cand = w.cand, *sup = w.sup, *scnt = w.scnt, *bip = w.bip;
-  uint32_t *pr = w.pr, *prc = w.prc, *rows = w.rows, *rcnt = w.rcnt;
-  const uint32_t tb = 256, tg = (N + tb - 1) / tb;
+#define PG_FUSION_WS                                                        \
+  const uint32_t N = f.n_base, K = f.gt_k, M = f.M_nn, R = f.rounds;        \
+  const uint32_t dim = f.dim, cap = 128u - M, wf = 2 * M + f.M_final;        \
+  const uint32_t nblk = (N + kScanBlock - 1) / kScanBlock;                  \
+  const FusionWs w = fusion_carve(f, ws);                                    \
+  uint32_t *owner = w.owner, *ccnt = w.ccnt, *tgt = w.tgt, *fcnt = w.fcnt;  \
+  uint32_t *deg = w.deg, *cursor = w.cursor, *off = w.off, *bcnt = w.bcnt;  \
+  uint32_t *scan_tmp = w.scan_tmp, *fwd = w.fwd, *srcs = w.srcs;            \
+  uint32_t *cand = w.cand, *sup = w.sup, *scnt = w.scnt, *bip = w.bip;      \
+  uint32_t *pr = w.pr, *prc = w.prc, *rows = w.rows, *rcnt = w.rcnt;        \
+  const uint32_t tb = 256, tg = (N + tb - 1) / tb;                          \
+  (void)K; (void)cap; (void)wf; (void)nblk; (void)owner; (void)tgt;          \
+  (void)fcnt; (void)deg; (void)cursor; (void)off; (void)fwd; (void)srcs;     \
+  (void)sup; (void)scnt; (void)pr; (void)prc; (void)rows; (void)rcnt;        \
+  (void)R; (void)dim; (void)scan_tmp; (void)cand; (void)ccnt
+
+// MatchSup rounds 1..R and FusionNN: the bipartite rows (bip, bcnt) in the
+// workspace.  Reads only base and gt, so it can run beside the GPU stage.
+hipError_t launch_fusion_bipartite(const FusionShape& f, const float* base,
+                                   const uint32_t* gt, uint32_t* ws,
+                                   hipStream_t s) {
+  if (f.n_base == 0) return hipSuccess;
+  PG_FUSION_WS;
   hipError_t e = hipSuccess;
   for (uint32_t r = 0; r < R && e == hipSuccess; ++r) {
     const uint32_t col = r + 1;
@@ -3180,12 +3188,22 @@ hipError_t launch_fusion(const FusionShape& f, const float* base,
   e = launch_fuse_rng(base, N, dim, cand, 256, ccnt, nullptr, N, R * M,
                       2 * M, kRngStrict, bip, 2 * M, bcnt, s);
   if (e != hipSuccess) return e;
-  // FusionFinal
+  return hipGetLastError();
+}
+
+// FusionFinal from the workspace's bipartite rows, the link rows and the
+// projection rows, as CSR.
+hipError_t launch_fusion_final(const FusionShape& f, const float* base,
+                               const uint32_t* proj, const uint32_t* link,
+                               uint32_t* ws, uint32_t* off_out,
+                               uint32_t* ids_out, hipStream_t s) {
+  if (f.n_base == 0) return hipSuccess;
+  PG_FUSION_WS;
   final_lists_kernel<<<tg, tb, 0, s>>>(bip, bcnt, 2 * M, link, f.M_link, proj,
                                        f.proj_deg, N, cand, 256, ccnt);
-  e = launch_fuse_rng(base, N, dim, cand, 256, ccnt, nullptr, N,
-                      2 * M + f.M_link + f.proj_deg, f.M_final, kRngStrict,
-                      pr, f.M_final, prc, s);
+  hipError_t e = launch_fuse_rng(base, N, dim, cand, 256, ccnt, nullptr, N,
+                                 2 * M + f.M_link + f.proj_deg, f.M_final,
+                                 kRngStrict, pr, f.M_final, prc, s);
   if (e != hipSuccess) return e;
   final_compose_kernel<<<tg, tb, 0, s>>>(bip, bcnt, 2 * M, pr, prc, f.M_final,
                                          N, rows, wf, rcnt);
@@ -3194,6 +3212,16 @@ hipError_t launch_fusion(const FusionShape& f, const float* base,
   scan_add_kernel<<<tg, tb, 0, s>>>(off_out, N, scan_tmp);
   rows_to_csr_kernel<<<tg, tb, 0, s>>>(rows, wf, off_out, N, ids_out);
   return hipGetLastError();
+}
+#undef PG_FUSION_WS
+
+hipError_t launch_fusion(const FusionShape& f, const float* base,
+                         const uint32_t* gt, const uint32_t* proj,
+                         const uint32_t* link, uint32_t* ws, uint32_t* off_out,
+                         uint32_t* ids_out, hipStream_t s) {
+  hipError_t e = launch_fusion_bipartite(f, base, gt, ws, s);
+  if (e != hipSuccess) return e;
+  return launch_fusion_final(f, base, proj, link, ws, off_out, ids_out, s);
 }
 
 }  // namespace pg

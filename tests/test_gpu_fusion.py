@@ -91,3 +91,48 @@ def test_gpu_fusion_rejects_out_of_range_shapes(cuda):
     hp = host_stage.make_params(n + 1, dim, n + 1, gt_k=gt.shape[1])
     with pytest.raises(_native.PgError, match="n_base"):
         _gpu_fusion(hp, base, gt, proj, link, torch)
+
+
+def test_bipartite_beside_gpu_stage_equals_host_stages(cuda):
+    """The bipartite stage on the context's second stream, beside the GPU
+    stage on its main stream, then FusionFinal on the GPU stage's own
+    outputs: the same lists as the host stages on those outputs, and as
+    pg_run_fusion."""
+    import torch
+    n, dim = 1200, 200
+    w = datagen.make_workload(n, dim, 64, device="cpu")
+    base = w["base"]
+    gt = np.ascontiguousarray(w["gt"].view(np.uint32))
+    hp = host_stage.make_params(n, dim, n, gt_k=gt.shape[1])
+    p = _native.make_params(n, dim, n, 32, 64)
+    width = host_stage.lib().pg_host_final_width(host_stage.ctypes.byref(hp))
+    base_d, gt_d = _dev(base, torch), _dev(gt, torch)
+    i32 = dict(dtype=torch.int32, device="cuda")
+    proj_d = torch.empty((n, p.proj_deg), **i32)
+    link_d = torch.empty((n, p.M), **i32)
+    outs = []
+    with _native.Context(p) as ctx:
+        with pytest.raises(_native.PgError, match="bipartite"):
+            ctx.run_fusion_final(hp, base_d, proj_d, link_d,
+                                 torch.empty(n + 1, **i32),
+                                 torch.empty(n * width, **i32))
+        for overlapped in (True, False, True):
+            off_d = torch.full((n + 1,), -7, **i32)
+            ids_d = torch.full((n * width,), -7, **i32)
+            if overlapped:
+                ctx.run_bipartite_async(hp, base_d, gt_d)
+                ctx.stage_device(base_d, gt_d, proj_d, link_d)
+                ctx.run_fusion_final(hp, base_d, proj_d, link_d, off_d, ids_d)
+            else:
+                ctx.stage_device(base_d, gt_d, proj_d, link_d)
+                ctx.run_fusion(hp, base_d, gt_d, proj_d, link_d, off_d, ids_d)
+            ctx.wait()
+            off = off_d.cpu().numpy().view(np.uint32)
+            outs.append((off, ids_d.cpu().numpy().view(np.uint32)[:off[-1]]))
+    proj = proj_d.cpu().numpy().view(np.uint32)
+    link = link_d.cpu().numpy().view(np.uint32)
+    bo, bi = host_stage.bipartite(hp, base, gt)
+    fo, fi = host_stage.fusion_final(hp, base, bo, bi, link, proj)
+    for off, ids in outs:
+        np.testing.assert_array_equal(off, fo)
+        np.testing.assert_array_equal(ids, fi)
