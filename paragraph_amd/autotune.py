@@ -162,6 +162,32 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             if t < best_pt:
                 best_pf, best_pt = split | scan, t
     best["tune_flags"] |= best_pf
+    # link-row merge in the locality order (bit 10, needs bit 9), timed on
+    # the reverse lists of the pruned rows of every base row
+    if best["tune_flags"] & 0x200:
+        pp = _native.PgParams.from_buffer_copy(p)
+        pp.tune_flags = (p.tune_flags & ~0x7BF) | best["tune_flags"]
+        off = torch.empty(n + 1, **i32)
+        srcs = torch.empty(n * p.M, **i32)
+        out = torch.empty((n, p.M), **i32)
+        with _native.Context(pp) as ctx:
+            ctx.run_prune_rows(base_d, full, p.M, pr, pc)
+            ctx.run_reverse(pr, pc, off, srcs)
+            ctx.sync()
+        mt = {}
+        for mo in (0, 0x400):
+            pp.tune_flags = ((p.tune_flags & ~0x7BF) | best["tune_flags"]
+                             | mo)
+            with _native.Context(pp) as ctx:
+                mt[mo] = _time(lambda: (ctx.run_merge(
+                    base_d, off, srcs, p.link_rev, pr, pc, p.M, True, out),
+                    ctx.sync()))
+            report.setdefault("merge", []).append(
+                {"order": bool(mo), "ms": round(1000 * mt[mo], 3)})
+            if log:
+                log(f"autotune merge order={bool(mo)}: {1000 * mt[mo]:.2f} ms")
+        if mt[0x400] < mt[0]:
+            best["tune_flags"] |= 0x400
     report["best"] = best
     report["seconds"] = round(time.perf_counter() - t_start, 2)
     if use_cache:
@@ -177,7 +203,7 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
 def apply(p, best):
     """Copy of p with the tuned knobs."""
     pp = _native.PgParams.from_buffer_copy(p)
-    pp.tune_flags = (p.tune_flags & ~0x3BF) | best["tune_flags"]
+    pp.tune_flags = (p.tune_flags & ~0x7BF) | best["tune_flags"]
     pp.table_size = best["table_size"]
     return pp
 

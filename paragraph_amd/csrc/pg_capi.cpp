@@ -152,11 +152,13 @@ uint32_t effective_table_size(const pg_params& p) {
 //   bit 9     persistent search takes its queries in a locality order
 //             (nodes grouped by nearest pivot row, one queue segment per
 //             XCD), and the link-row prune sweep its nodes; same rows
+//   bit 10    with bit 9, the merge takes its nodes in that order too
 constexpr uint32_t kTunePersistent = 0x20u;
 constexpr uint32_t kTuneGraph = 0x40u;
 constexpr uint32_t kTuneSplit = 0x80u;
 constexpr uint32_t kTuneScanPrune = 0x100u;
 constexpr uint32_t kTuneOrder = 0x200u;
+constexpr uint32_t kTuneOrderMerge = 0x400u;
 uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
 
 // Bound factor of the BF16 screen against the exact FP32 chain:
@@ -272,7 +274,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_merge(d_base, N, p.dim, c->off, c->srcs, p.proj_rev,
                              c->pruned, c->pcnt, p.proj_pr, p.proj_deg, 0,
-                             d_proj, c->stats, nullptr, s));
+                             d_proj, c->stats, nullptr, nullptr, s));
   PG_TRY(record(c, ev++));
   const uint32_t* graph = d_proj;
   uint32_t gdeg = p.proj_deg;
@@ -305,7 +307,10 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_merge(d_base, N, p.dim, c->off, c->srcs, p.link_rev,
                                c->pruned, c->pcnt, p.M, p.M, 1, d_link,
-                               c->stats, scp, s));
+                               c->stats, scp,
+                               (p.tune_flags & kTuneOrderMerge) ? sorder
+                                                                : nullptr,
+                               s));
     PG_TRY(record(c, ev++));
     graph = d_link;
     gdeg = p.M;
@@ -759,9 +764,13 @@ int pg_run_merge(pg_ctx* c, const float* d_base, const uint32_t* d_off,
   PG_HIP(c, hipSetDevice(c->device));
   const pg::ScreenArgs sc = screen_args(c);
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
+  const uint32_t* sorder = nullptr;
+  if (c->p.tune_flags & kTuneOrderMerge)
+    PG_TRY(search_order(c, d_base, c->p.n_base, &sorder));
   PG_HIP(c, pg::launch_merge(d_base, c->p.n_base, c->p.dim, d_off, d_srcs, cap,
                              d_pruned, d_pcnt, mp, deg, strict, d_out,
-                             c->stats, c->base16 ? &sc : nullptr, c->stream));
+                             c->stats, c->base16 ? &sc : nullptr, sorder,
+                             c->stream));
   return PG_OK;
 }
 

@@ -47,7 +47,8 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
                         uint32_t cap, const uint32_t* pruned,
                         const uint32_t* pcnt, uint32_t mp, uint32_t deg,
                         int strict, uint32_t* out, unsigned long long* stats,
-                        const ScreenArgs* sc, hipStream_t s);
+                        const ScreenArgs* sc, const uint32_t* order,
+                        hipStream_t s);
 
 size_t search_smem_bytes(uint32_t table_size, uint32_t lp2, uint32_t kpkd,
                          uint32_t Kp);

@@ -1204,7 +1204,8 @@ __global__ __launch_bounds__(64) void merge_kernel(
     uint32_t cap, const uint32_t* __restrict__ pruned,
     const uint32_t* __restrict__ pcnt, uint32_t mp, uint32_t deg,
     int strict, uint32_t* __restrict__ out,
-    unsigned long long* __restrict__ stats, ScreenArgs sc) {
+    unsigned long long* __restrict__ stats, ScreenArgs sc,
+    const uint32_t* __restrict__ order) {
   __shared__ uint32_t buf[2 * kSegChunk];
   __shared__ uint64_t skey[128];
   __shared__ uint32_t sid[128];
@@ -1212,7 +1213,9 @@ __global__ __launch_bounds__(64) void merge_kernel(
   __shared__ float2 kept_nr[NCH ? 128 : 1];
   __shared__ float gs[NCH ? 16 * (NKT * 16 + 16) : 1];
   __shared__ uint8_t added[128];
-  const uint32_t c = blockIdx.x;
+  // nodes in the locality order when one is given (one output row per
+  // node, so the launch order changes no result)
+  const uint32_t c = order ? order[blockIdx.x] : blockIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t s0 = off[c], seg = off[c + 1] - s0;
   const uint32_t r = seg < cap ? seg : cap;
@@ -2130,14 +2133,16 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
                         uint32_t cap, const uint32_t* pruned,
                         const uint32_t* pcnt, uint32_t mp, uint32_t deg,
                         int strict, uint32_t* out, unsigned long long* stats,
-                        const ScreenArgs* sc, hipStream_t s) {
+                        const ScreenArgs* sc, const uint32_t* order,
+                        hipStream_t s) {
   if (n_base == 0) return hipSuccess;
   const ScreenArgs scv =
       sc ? *sc : ScreenArgs{nullptr, nullptr, nullptr, nullptr, 0, 0.0f, 0.0f, 0.0f};
 #define PG_MERGE(NC, NK, SP)                                                 \
   merge_kernel<NC, NK, SP><<<n_base, 64, 0, s>>>(base, n_base, dim, off,     \
                                                  srcs, cap, pruned, pcnt, mp, \
-                                                 deg, strict, out, stats, scv)
+                                                 deg, strict, out, stats, scv, \
+                                                 order)
 #define PG_MERGE_K(NC, SP)                    \
   do {                                        \
     if (deg <= 32) PG_MERGE(NC, 2, SP);       \

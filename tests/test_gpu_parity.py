@@ -271,16 +271,18 @@ def test_locality_order_search_matches_oracle(n_query, cuda):
 
 @pytest.mark.parametrize("prec", [_native.PG_FP32, _native.PG_BF16])
 def test_locality_order_build_equals_plain_build(prec, cuda):
-    """A whole build with the locality order (eager, and captured in a
-    hipGraph and replayed) gives the same rows as without it; the FP32
-    rows also equal the oracle."""
+    """A whole build with the locality order for the search and the prune
+    sweep (bit 9) and the merge (bit 10), eager and captured in a hipGraph
+    and replayed, gives the same rows as without it; the FP32 rows also
+    equal the oracle."""
     import torch
     w = workload("d200_m32_l100")
     p0 = w["params"]
     n = p0.n_base
     base_d, gt_d = to_dev(w["base"], torch), to_dev(w["gt_u32"], torch)
     outs = []
-    for tf in (0x20, 0x20 | 0x200, 0x20 | 0x200 | 0x40):
+    for tf in (0x20, 0x20 | 0x200, 0x20 | 0x200 | 0x40,
+               0x20 | 0x200 | 0x400, 0x20 | 0x200 | 0x400 | 0x40):
         p = _native.make_params(n, p0.dim, n, p0.M, p0.Km, precision=prec,
                                 tune_flags=tf)
         proj_d = torch.full((n, 16), -7, dtype=torch.int32, device=cuda)
