@@ -96,7 +96,7 @@ typedef struct pg_params {
                            the persistent search takes its queries in a
                            locality order (nodes grouped by nearest pivot
                            row, one queue segment per XCD), and the
-                           link-row prune sweep its nodes; bit 10 (0x400),
+                           link-row prune its nodes; bit 10 (0x400),
                            with bit 9: the link-row merge takes its nodes
                            in that order too (same rows) */
   uint32_t reserved[7];

@@ -106,32 +106,37 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             log(f"autotune search {kn}: {1000 * t:.2f} ms")
         if t < best_t:
             best, best_t = dict(kn), t
-    # grid shape on all base rows, with the best sampled gather / table
-    # knobs: one workgroup per query, the persistent grid, and the
-    # persistent grid taking its queries in the locality order (their gains
-    # are cache reuse and load balance across the whole query range, which
-    # a sample does not show)
+    # gather batch and grid shape on all base rows, with the best sampled
+    # visited-table size: the persistent grid, and the persistent grid
+    # taking its queries in the locality order (cache reuse and load
+    # balance across the whole query range do not show on a sample, and a
+    # sample's gather ranking was seen to flip on the full run)
     if best is not None:
         full_t = {}
-        for grid in (0, 0x20, 0x220):
-            pp = _native.PgParams.from_buffer_copy(p)
-            pp.tune_flags = ((p.tune_flags & ~0x23F)
-                             | (best["tune_flags"] & 0x1F) | grid)
-            pp.table_size = best["table_size"]
-            out = torch.empty((n, pp.topk), **i32)
-            with _native.Context(pp) as ctx:
-                full_t[grid] = _time(lambda: (ctx.run_search(
-                    base_d, base_d, link, out), ctx.sync()))
-            del out
-            report["search"].append({"tune_flags": int(pp.tune_flags),
-                                     "table_size": best["table_size"],
-                                     "rows": n,
-                                     "ms": round(1000 * full_t[grid], 3)})
-            if log:
-                log(f"autotune search grid={grid:#x} (all rows): "
-                    f"{1000 * full_t[grid]:.2f} ms")
-        g = min(full_t, key=full_t.get)
-        best["tune_flags"] = (best["tune_flags"] & ~0x220) | g
+        gathers = sorted({kn["tune_flags"] & 0x1F for kn in candidates(p)})
+        for g in gathers:
+            for grid in (0x20, 0x220):
+                pp = _native.PgParams.from_buffer_copy(p)
+                pp.tune_flags = (p.tune_flags & ~0x23F) | g | grid
+                pp.table_size = best["table_size"]
+                if _native.check_params(pp)[0] != 0:
+                    continue
+                out = torch.empty((n, pp.topk), **i32)
+                with _native.Context(pp) as ctx:
+                    full_t[g | grid] = _time(lambda: (ctx.run_search(
+                        base_d, base_d, link, out), ctx.sync()))
+                del out
+                report["search"].append({"tune_flags": int(pp.tune_flags),
+                                         "table_size": best["table_size"],
+                                         "rows": n,
+                                         "ms": round(1000 * full_t[g | grid],
+                                                     3)})
+                if log:
+                    log(f"autotune search gather={g} grid={grid:#x} (all "
+                        f"rows): {1000 * full_t[g | grid]:.2f} ms")
+        if full_t:
+            gg = min(full_t, key=full_t.get)
+            best["tune_flags"] = (best["tune_flags"] & ~0x23F) | gg
     # prune variants on the search rows of every base row (the prune's cost
     # depends on cache reuse across the whole node range, so a sample with
     # empty rows elsewhere misranks them): kept-major sweep or

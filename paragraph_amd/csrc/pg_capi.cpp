@@ -151,7 +151,7 @@ uint32_t effective_table_size(const pg_params& p) {
 //             order) instead of the kept-major sweep
 //   bit 9     persistent search takes its queries in a locality order
 //             (nodes grouped by nearest pivot row, one queue segment per
-//             XCD), and the link-row prune sweep its nodes; same rows
+//             XCD), and the link-row prune its nodes; same rows
 //   bit 10    with bit 9, the merge takes its nodes in that order too
 constexpr uint32_t kTunePersistent = 0x20u;
 constexpr uint32_t kTuneGraph = 0x40u;

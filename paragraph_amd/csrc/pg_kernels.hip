@@ -423,13 +423,15 @@ __global__ __launch_bounds__(64) void prune_kernel(
     const uint32_t* __restrict__ gt, const uint32_t* __restrict__ winner,
     uint32_t gt_k, uint32_t mp, uint32_t* __restrict__ pruned,
     uint32_t* __restrict__ pcnt, unsigned long long* __restrict__ stats,
-    ScreenArgs sc) {
+    ScreenArgs sc, const uint32_t* __restrict__ order) {
   __shared__ uint64_t skey[DP2];
   __shared__ uint32_t sid[DP2];
   __shared__ uint32_t kept[128];
   __shared__ float2 kept_nr[NCH ? 128 : 1];
   __shared__ float gs[NCH ? 16 * (NKT * 16 + 16) : 1];
-  const uint32_t b = blockIdx.x;
+  // nodes in the locality order when one is given (rows are per node, so
+  // the launch order changes no result)
+  const uint32_t b = order ? order[blockIdx.x] : blockIdx.x;
   const uint32_t lane = lane_id();
   __shared__ uint16_t vpos[DP2];
   const uint32_t w = rows ? 0u : winner[b];
@@ -2072,7 +2074,7 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
 #define PG_PRUNE(DP, NC, NK, SP)                                           \
   prune_kernel<DP, NC, NK, SP><<<n_base, 64, 0, s>>>(                      \
       base, n_base, dim, rows, D, gt, winner, gt_k, mp, pruned, pcnt, stats, \
-      scv)
+      scv, rows ? order : nullptr)
 #define PG_PRUNE_D(NC, NK, SP)                        \
   do {                                                \
     if (D <= 256) PG_PRUNE(256, NC, NK, SP);          \
