@@ -167,6 +167,30 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             if t < best_pt:
                 best_pf, best_pt = split | scan, t
     best["tune_flags"] |= best_pf
+    # the locality order (bit 9) also orders the link-row prune: decide it
+    # on search + prune time with the chosen gather batch and prune variant
+    if best is not None and full_t:
+        g = best["tune_flags"] & 0x1F
+        tot = {}
+        for grid in (0x20, 0x220):
+            if (g | grid) not in full_t:
+                continue
+            pp = _native.PgParams.from_buffer_copy(p)
+            pp.tune_flags = ((p.tune_flags & ~0x3BF) | g | grid | best_pf)
+            with _native.Context(pp) as ctx:
+                t = _time(lambda: (ctx.run_prune_rows(base_d, full, p.M, pr,
+                                                      pc), ctx.sync()))
+            tot[grid] = full_t[g | grid] + t
+            report["prune"].append({"order": bool(grid & 0x200),
+                                    "split": bool(best_pf & 0x80),
+                                    "scan": bool(best_pf & 0x100),
+                                    "ms": round(1000 * t, 3)})
+            if log:
+                log(f"autotune order={bool(grid & 0x200)}: search + prune "
+                    f"{1000 * tot[grid]:.2f} ms")
+        if tot:
+            gbest = min(tot, key=tot.get)
+            best["tune_flags"] = (best["tune_flags"] & ~0x220) | gbest
     # link-row merge in the locality order (bit 10, needs bit 9), timed on
     # the reverse lists of the pruned rows of every base row
     if best["tune_flags"] & 0x200:
