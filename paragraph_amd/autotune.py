@@ -215,7 +215,9 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
                 {"order": bool(mo), "ms": round(1000 * mt[mo], 3)})
             if log:
                 log(f"autotune merge order={bool(mo)}: {1000 * mt[mo]:.2f} ms")
-        if mt[0x400] < mt[0]:
+        # a clear gain only: on the 1M x 200 rows the two tie here while the
+        # ordered merge ran slower in the first round of the pipeline
+        if mt[0x400] < 0.97 * mt[0]:
             best["tune_flags"] |= 0x400
     report["best"] = best
     report["seconds"] = round(time.perf_counter() - t_start, 2)

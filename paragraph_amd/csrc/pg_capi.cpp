@@ -65,6 +65,8 @@ struct pg_ctx {
   uint32_t* order = nullptr;   // locality order of the search queries
   uint32_t* ogroup = nullptr;  // its pivot group per node
   uint32_t* ocnt = nullptr;    // its group counters
+  const void* order_rows = nullptr;  // rows and count the order was built
+  uint32_t order_n = 0;              // for (reused by the pg_run_* calls)
   uint32_t* fuse_ws = nullptr;  // host stages on the GPU (pg_run_fusion)
   uint64_t fuse_words = 0;
   hipStream_t aux = nullptr;    // second stream: the bipartite stage beside
@@ -239,15 +241,23 @@ int prepare_screen(pg_ctx* c, const float* d_base) {
 
 // Locality order of `n` query rows for the persistent search (tune_flags
 // bit 9): *out = the order, or nullptr when the order is off, the grid is
-// not persistent or n exceeds the order's buffers (n_base rows).
+// not persistent or n exceeds the order's buffers (n_base rows).  reuse:
+// keep the order already built for the same rows pointer and count (the
+// single-stage entry points; a build recomputes it).  Any permutation gives
+// the same results, so an order built from rows that changed since only
+// costs speed.
 int search_order(pg_ctx* c, const float* d_rows, uint32_t n,
-                 const uint32_t** out) {
+                 const uint32_t** out, bool reuse = false) {
   *out = nullptr;
   if (!c->order || !(c->p.tune_flags & kTunePersistent) || n == 0 ||
       n > c->p.n_base)
     return PG_OK;
-  PG_HIP(c, pg::launch_locality_order(d_rows, n, c->p.dim, c->ogroup, c->ocnt,
-                                      c->order, c->stream));
+  if (!(reuse && c->order_rows == d_rows && c->order_n == n)) {
+    PG_HIP(c, pg::launch_locality_order(d_rows, n, c->p.dim, c->ogroup,
+                                        c->ocnt, c->order, c->stream));
+    c->order_rows = d_rows;
+    c->order_n = n;
+  }
   *out = c->order;
   return PG_OK;
 }
@@ -733,7 +743,7 @@ int pg_run_prune(pg_ctx* c, const float* d_base, const uint32_t* d_rows,
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
   // link rows take nodes in the locality order when the context has one
   const uint32_t* sorder = nullptr;
-  if (d_rows) PG_TRY(search_order(c, d_base, c->p.n_base, &sorder));
+  if (d_rows) PG_TRY(search_order(c, d_base, c->p.n_base, &sorder, true));
   PG_HIP(c, pg::launch_prune(d_base, c->p.n_base, c->p.dim, d_rows, D, d_gt,
                              d_winner, gt_k, mp, d_pruned, d_pcnt, c->stats,
                              c->base16 ? &sc : nullptr,
@@ -766,7 +776,7 @@ int pg_run_merge(pg_ctx* c, const float* d_base, const uint32_t* d_off,
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
   const uint32_t* sorder = nullptr;
   if (c->p.tune_flags & kTuneOrderMerge)
-    PG_TRY(search_order(c, d_base, c->p.n_base, &sorder));
+    PG_TRY(search_order(c, d_base, c->p.n_base, &sorder, true));
   PG_HIP(c, pg::launch_merge(d_base, c->p.n_base, c->p.dim, d_off, d_srcs, cap,
                              d_pruned, d_pcnt, mp, deg, strict, d_out,
                              c->stats, c->base16 ? &sc : nullptr, sorder,
@@ -786,7 +796,7 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
   PG_HIP(c, hipSetDevice(c->device));
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
   const uint32_t* sorder = nullptr;
-  PG_TRY(search_order(c, d_queries, n_query, &sorder));
+  PG_TRY(search_order(c, d_queries, n_query, &sorder, true));
   PG_HIP(c, pg::launch_search(d_queries, n_query, d_base,
                               c->base8 ? nullptr : c->base16, c->base8,
                               c->scale8, c->dpad,

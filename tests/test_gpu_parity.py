@@ -262,7 +262,7 @@ def test_locality_order_search_matches_oracle(n_query, cuda):
                             tune_flags=0x20 | 0x200)
     out_d = torch.full((n_query, p.topk), -7, dtype=torch.int32, device=cuda)
     with _native.Context(p) as ctx:
-        for _ in range(2):  # the order is rebuilt per call
+        for _ in range(2):  # the second call reuses the order
             ctx.run_search(q_d, base_d, graph_d, out_d)
             ctx.sync()
     ref = cpu_ref.search(queries, w["base"], graph, p.Km, p.Kp, p.Kd, p.topk)
