@@ -65,6 +65,26 @@ def test_gpu_fusion_equals_host_stages(n, dim, nc, norm, cuda):
     np.testing.assert_array_equal(ids, fi)
 
 
+def test_gpu_fusion_ragged_zipf_gt(cuda):
+    """Zipf query components, per-query gt lengths in [16, 128] padded with
+    -1, dim 512 normalised, 48-wide link rows (the skewed workload's shape)."""
+    import torch
+    n, dim = 1200, 512
+    w = datagen.make_workload(n, dim, 64, normalize=True, zipf=1.1,
+                              gt_len_range=(16, 128), device="cpu")
+    base = w["base"]
+    gt = np.ascontiguousarray(w["gt"].view(np.uint32))
+    rng = np.random.default_rng(9)
+    link = rng.integers(0, n, (n, 48)).astype(np.uint32)
+    proj = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    hp = host_stage.make_params(n, dim, n, gt_k=gt.shape[1])
+    off, ids = _gpu_fusion(hp, base, gt, proj, link, torch)
+    bo, bi = host_stage.bipartite(hp, base, gt)
+    fo, fi = host_stage.fusion_final(hp, base, bo, bi, link, proj)
+    np.testing.assert_array_equal(off, fo)
+    np.testing.assert_array_equal(ids, fi)
+
+
 def test_gpu_fusion_small_degrees_equal_oracle(cuda):
     """M_nn 60 (reverse cap 68, overflowing), 4 rounds, M_final 12 (the
     unsigned room: bipartite lists longer than M_final keep growing)."""

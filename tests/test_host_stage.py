@@ -344,6 +344,30 @@ def test_product_equals_oracle(n, dim, nc, norm):
     np.testing.assert_array_equal(fi, gi)
 
 
+def test_product_equals_oracle_ragged_zipf_gt():
+    """The skewed workload's shape: Zipf query components, per-query gt
+    lengths in [16, 128] padded with -1 (MatchSup's rows end at the first
+    id >= N), L2-normalised dim 512."""
+    n, dim = 1200, 512
+    w = datagen.make_workload(n, dim, 64, normalize=True, zipf=1.1,
+                              gt_len_range=(16, 128), device="cpu")
+    base = w["base"]
+    gt = np.ascontiguousarray(w["gt"].view(np.uint32))
+    assert (gt == TOMB).any()
+    p = host_stage.make_params(n, dim, n, gt_k=gt.shape[1])
+    bo, bi = host_stage.bipartite(p, base, gt)
+    oo, oi = cpu_ref.bipartite(base, gt, p.M_nn, p.rounds)
+    np.testing.assert_array_equal(bo, oo)
+    np.testing.assert_array_equal(bi, oi)
+    rng = np.random.default_rng(9)
+    link = rng.integers(0, n, (n, 48)).astype(np.uint32)
+    proj = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    fo, fi = host_stage.fusion_final(p, base, bo, bi, link, proj)
+    go, gi = cpu_ref.fusion_final(base, oo, oi, link, proj, p.M_final)
+    np.testing.assert_array_equal(fo, go)
+    np.testing.assert_array_equal(fi, gi)
+
+
 def test_product_small_degrees_and_threads():
     """M_nn / M_final small enough that the reverse cap, the strict merge
     and FusionFinal's unsigned room all occur; 1 and 8 threads agree."""
