@@ -156,3 +156,30 @@ def test_bipartite_beside_gpu_stage_equals_host_stages(cuda):
     for off, ids in outs:
         np.testing.assert_array_equal(off, fo)
         np.testing.assert_array_equal(ids, fi)
+
+
+def test_gpu_fusion_hub_with_long_reverse_list(cuda):
+    """Every query's gt row holds base 7 and most hold base 11, so their
+    MatchSup forward lists (non-strict, filled to M_nn) keep them: reverse
+    segments of thousands of sources, past the merge's 512-source LDS chunk,
+    cut to the cap of the lowest query ids (rule C6)."""
+    import torch
+    n, dim, K = 3000, 64, 32
+    rng = np.random.default_rng(21)
+    base = rng.standard_normal((n, dim)).astype(np.float32)
+    gt = np.empty((n, K), np.uint32)
+    for q in range(n):
+        gt[q] = rng.permutation(n)[:K]
+    gt[:, 5] = 7
+    gt[::2, 9] = 11
+    link = rng.integers(0, n, (n, 32)).astype(np.uint32)
+    proj = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    hp = host_stage.make_params(n, dim, n, gt_k=K, rounds=4)
+    off, ids = _gpu_fusion(hp, base, gt, proj, link, torch)
+    bo, bi = host_stage.bipartite(hp, base, gt)
+    fo, fi = host_stage.fusion_final(hp, base, bo, bi, link, proj)
+    np.testing.assert_array_equal(off, fo)
+    np.testing.assert_array_equal(ids, fi)
+    # the hub's supply list really is a capped merge (> M_nn candidates)
+    so, si = host_stage.match_sup(hp, base, gt, 1)
+    assert so[8] - so[7] > 0
