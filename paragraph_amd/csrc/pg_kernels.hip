@@ -1197,7 +1197,11 @@ __global__ void scan_add_kernel(uint32_t* __restrict__ out, uint32_t n,
 // in ascending id (C2), then c's pruned list.  Stable distance sort, list[0]
 // kept unconditionally, strict RNG to `deg`, non-strict fill when !strict
 // (Gbuilder.cuh:1559-1594), tomb fill.
-constexpr uint32_t kSegChunk = 512;
+// Sources are selected in LDS chunks of kSegChunk ids (longer segments take
+// several sort passes): 256 keeps the screened kernel at 8.3 KB of LDS, four
+// waves per SIMD instead of three (BF16 merge 4.8 / 5.1 -> 4.2 / 4.5 ms per
+// round on 1M x 200).
+constexpr uint32_t kSegChunk = 256;
 
 template <int NCH, int NKT, bool kSplit>
 __global__ __launch_bounds__(64) void merge_kernel(
