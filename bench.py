@@ -188,10 +188,11 @@ def main():
                          "rows, 200000-row shards above)")
     ap.add_argument("--end-to-end", action="store_true",
                     help="also time the whole index build of the value "
-                         "path: GPU stage with the host bipartite stage "
-                         "(MatchSup rounds + FusionNN) on a CPU thread "
-                         "beside it, then FusionFinal and the entry point "
-                         "(reported in extra, never as value)")
+                         "path, twice: with the host stages on the host "
+                         "(bipartite stage on a CPU thread beside the GPU "
+                         "stage, then FusionFinal and the entry point) and "
+                         "with them on the GPU (bipartite stage on a second "
+                         "stream); reported in extra, never as value")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run a 1 GiB device copy first (counter calibration)")
     args = ap.parse_args()
