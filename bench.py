@@ -256,6 +256,23 @@ def main():
     # the evaluation searcher: reference mapping at the workload's L
     p_eval = _native.make_params(n, dim, n, M, L)
     eval_ctx = _native.Context(p_eval)
+    # the final index of a build (its GPU stage rows through the host stages
+    # on the GPU, on the evaluation context)
+    from paragraph_amd import host_stage
+    hp_final = host_stage.make_params(n, dim, gt_u32.shape[0],
+                                      gt_k=gt_u32.shape[1])
+    fin_width = int(host_stage.lib().pg_host_final_width(
+        host_stage.ctypes.byref(hp_final)))
+    fin_off_d = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    fin_ids_d = torch.empty(n * fin_width, dtype=torch.int32, device=dev)
+
+    def final_index(proj, link):
+        eval_ctx.run_fusion(hp_final, base_d, gt_d, proj, link, fin_off_d,
+                            fin_ids_d)
+        eval_ctx.sync()
+        off = fin_off_d.cpu().numpy().view(np.uint32)
+        return off, fin_ids_d[:int(off[-1])].cpu().numpy().view(np.uint32)
+
     truth = None
     if not args.no_recall:
         eq_d, truth = exact_top10(w["eval_queries"])
@@ -265,9 +282,10 @@ def main():
         tq_d, tq_truth = exact_top10(datagen.tune_queries(args.config,
                                                           args.n_tune))
 
-        def tune_recall(link):
-            return recall_at_10(eval_ctx, tq_d, base_d, link, tq_truth,
-                                p_eval.topk)
+        # the screened beam is judged on the recall of the index it builds
+        def tune_recall(proj, link):
+            return final_index_recall(final_index(proj, link), base_d, tq_d,
+                                      tq_truth, n, dim)
     ref_tune_recall = None
     if tune_beam and "fp32" not in precs:
         # the FP32 build's recall on the tuning queries is the reference
@@ -277,7 +295,7 @@ def main():
             lk_ = torch.empty((n, M), dtype=torch.int32, device=dev)
             c32.stage_device(base_d, gt_d, pr_, lk_)
             c32.wait()
-            ref_tune_recall = tune_recall(lk_)
+            ref_tune_recall = tune_recall(pr_, lk_)
             del pr_, lk_
     # FP32 first: its recall anchors the beam tuning of the screened builds
     order = (["fp32"] if "fp32" in precs else []) + screened_req
@@ -335,12 +353,17 @@ def main():
         log(f"{prec}: {ms:.1f} ms/step, {r['value']:.0f} pts/s; stage ms "
             f"{ {k: round(v, 1) for k, v in detail['stage_ms'].items()} }")
         if prec == "fp32" and tune_beam:
-            ref_tune_recall = tune_recall(link_d)
+            ref_tune_recall = tune_recall(proj_d, link_d)
         if truth is not None:
             r["recall_at_10"] = round(recall_at_10(
                 eval_ctx, eq_d, base_d, link_d, truth, p_eval.topk), 4)
             log(f"{prec}: recall@10 of the link graph "
                 f"({truth.shape[0]} held-out queries) = {r['recall_at_10']}")
+            r["final_index_recall_at_10"] = final_index_recall(
+                final_index(proj_d, link_d), base_d, eq_d, truth, n, dim)
+            log(f"{prec}: recall@10 of the final index (reference "
+                "evaluation searcher: Km 128, Kp 6, Kd 64) = "
+                f"{r['final_index_recall_at_10']}")
         if tuned is not None:
             r["autotune"] = tuned
         r["tune_flags"] = int(p.tune_flags)
@@ -380,6 +403,7 @@ def main():
     if args.end_to_end:
         end_to_end = time_end_to_end(results, precs, w, gt_u32, base_d, gt_d,
                                      n, M, log)
+        end_to_end.pop("_final", None)
 
     # ---- CPU baseline: the oracle on a bounded node sample
     cpu = None
@@ -422,6 +446,9 @@ def main():
             extra[f"recall_delta_pt_{sp}"] = round(
                 100 * (results[sp]["recall_at_10"] -
                        results["fp32"]["recall_at_10"]), 2)
+            extra[f"final_index_recall_delta_pt_{sp}"] = round(
+                100 * (results[sp]["final_index_recall_at_10"] -
+                       results["fp32"]["final_index_recall_at_10"]), 2)
         a = results["fp32"]["link"].cpu().numpy()
         b = results[sp]["link"].cpu().numpy()
         # per-node neighbour-set overlap of the link rows (first 20k nodes)
@@ -456,9 +483,9 @@ def main():
                        "prune)"}.get(head_prec, ""))
                    + (f"; screened beam L={head['beam']['L']}, re-rank "
                       f"width {head['beam']['topk']}"
-                      + (" (autotuned: fastest L within min(0.3 pt, 5%) of "
-                         "the FP32 build's recall@10 on separate tuning "
-                         "queries)"
+                      + (" (autotuned: fastest L whose final-index "
+                         "recall@10 is within min(0.3 pt, 5%) of the FP32 "
+                         "build's, on separate tuning queries)"
                          if tune_beam else "")
                       if head_prec != "fp32" else ""),
                    "n_base": n, "dim": dim, "M": M, "L": L,
@@ -556,7 +583,29 @@ def time_end_to_end(results, precs, w, gt_u32, base_d, gt_d, n, M, log):
                  np.array_equal(off_g, off) and np.array_equal(ids_g, ids)
                  and eps["ep"] == ep)}
     log(f"end-to-end build, host stages on the GPU ({prec}): {r_gpu}")
-    return {"host_stages_on_host": r, "host_stages_on_gpu": r_gpu}
+    return {"host_stages_on_host": r, "host_stages_on_gpu": r_gpu,
+            "_final": (off, ids)}
+
+
+def final_index_recall(final, base_d, eq_d, truth, n, dim):
+    """recall@10 of the final index with the reference's evaluation
+    searcher knobs (search_shared_hashtable, Gsearcher.cuh:108-296: Km 128,
+    Kp 6, Kd 64, reset every 4) over its lists padded to rows."""
+    import torch
+    from paragraph_amd import _native, io
+    off, ids = final
+    width = max(64, int(np.diff(off.astype(np.int64)).max()))
+    rows_d = torch.from_numpy(io.csr_to_rows(off, ids, width).view(
+        np.int32)).to(base_d.device)
+    p = io.eval_search_params(n, dim, eq_d.shape[0])
+    out_d = torch.empty((eq_d.shape[0], p.topk), dtype=torch.int32,
+                        device=base_d.device)
+    with _native.Context(p) as ctx:
+        ctx.run_search(eq_d, base_d, rows_d, out_d)
+        ctx.sync()
+    found = out_d[:, :10].cpu().numpy()
+    hits = sum(len(set(a) & set(b)) for a, b in zip(found, truth))
+    return round(hits / truth.size, 4)
 
 
 def distance_error(ctxs, w, base_d, n_sample=2000):

@@ -259,7 +259,7 @@ def recall_floor(ref_recall, tol_pt=0.5, margin_pt=0.2, rel_tol=0.05):
 def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
               margin_pt=0.2, rel_tol=0.05, reps=2, log=None):
     """Screened builds: the fastest beam width L whose recall@10
-    (recall_fn(link_d), scored on tuning queries) is at least
+    (recall_fn(proj_d, link_d), scored on tuning queries) is at least
     ref_recall - min((tol_pt - margin_pt) / 100, rel_tol * ref_recall): an
     absolute tolerance in points, capped at a relative loss for workloads
     whose recall is itself small.  Returns (params, report)."""
@@ -284,7 +284,7 @@ def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
         with _native.Context(pp) as ctx:
             t = _time(lambda: (ctx.stage_device(base_d, gt_d, proj, link),
                                ctx.wait()), reps=reps)
-        rec = recall_fn(link)
+        rec = recall_fn(proj, link)
         ok = rec >= floor
         report["tried"].append({"L": L, "ms": round(1000 * t, 2),
                                 "recall_at_10": round(rec, 4), "ok": ok})

@@ -76,3 +76,34 @@ def load_index(path):
     if pos != raw.shape[0]:
         raise ValueError("trailing data in index file")
     return ep, lists
+
+
+def csr_to_rows(off, ids, width=None, tomb=0xFFFFFFFF):
+    """CSR lists -> fixed-width rows padded with tomb (uint32), the graph
+    layout the searcher reads (max_degree stride, Gsearcher.cuh:108-296).
+    width defaults to the longest list."""
+    off = np.asarray(off, dtype=np.int64)
+    ids = np.asarray(ids).view(np.uint32) if np.asarray(ids).dtype == \
+        np.int32 else np.asarray(ids, dtype=np.uint32)
+    n = off.shape[0] - 1
+    lens = np.diff(off)
+    if width is None:
+        width = int(lens.max(initial=1))
+    if (lens > width).any():
+        raise ValueError("a list is longer than width")
+    rows = np.full((n, width), tomb, np.uint32)
+    col = np.arange(ids.shape[0], dtype=np.int64) - np.repeat(off[:-1], lens)
+    rows[np.repeat(np.arange(n), lens), col] = ids
+    return rows
+
+
+def eval_search_params(n_base, dim, n_query, Km=128, Kp=6, Kd=64, topk=128,
+                       reset_iter=4):
+    """Build-parameter block for the reference's evaluation searcher
+    (search_shared_hashtable's defaults, Gsearcher.cuh:108-113: Km 128,
+    Kp 6, Kd 64, reset every 4 iterations) over an index of row width >= Kd;
+    only pg_run_search reads it."""
+    from . import _native
+    return _native.make_params(n_base, dim, n_query, 64, Km, Kp=Kp, Kd=Kd,
+                               topk=topk, reset_iter=reset_iter, rounds=1,
+                               proj_deg=max(Kd, 16), table_size=0)

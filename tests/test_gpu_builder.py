@@ -61,3 +61,33 @@ def test_build_index_matches_oracle(host_stages, cuda, tmp_path):
     assert ep_f == ep
     assert all(np.array_equal(lists[i], fi[fo[i]:fo[i + 1]])
                for i in range(pr["n_base"]))
+
+
+def test_eval_search_over_final_index_matches_oracle(cuda):
+    """The reference's evaluation searcher (search_shared_hashtable:
+    Km 128, Kp 6, Kd 64, reset every 4) over the final index, as padded
+    rows, equals the oracle's search over the same rows."""
+    import torch
+    from oracle import cpu_ref
+    from paragraph_amd import _native, io
+    g = golden_io.load_stage("stage_d200")
+    pr = g["params"]
+    n, dim = pr["n_base"], pr["dim"]
+    b = Builder(g["base"], g["gt"].view(np.int32))
+    ep, off, ids = b.BuildIndex(M=pr["M"], L=pr["Km"])
+    rows = io.csr_to_rows(off, ids, max(64, int(np.diff(off.astype(
+        np.int64)).max())))
+    rng = np.random.default_rng(4)
+    q = (g["base"][rng.permutation(n)[:200]] +
+         0.1 * rng.standard_normal((200, dim))).astype(np.float32)
+    p = io.eval_search_params(n, dim, q.shape[0])
+    out_d = torch.full((q.shape[0], p.topk), -7, dtype=torch.int32,
+                       device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.run_search(torch.from_numpy(q).to(cuda),
+                       torch.from_numpy(g["base"]).to(cuda),
+                       torch.from_numpy(rows.view(np.int32)).to(cuda), out_d)
+        ctx.sync()
+    ref = cpu_ref.search(q, g["base"], rows, p.Km, p.Kp, p.Kd, p.topk,
+                         reset_iter=p.reset_iter)
+    np.testing.assert_array_equal(out_d.cpu().numpy().view(np.uint32), ref)
