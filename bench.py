@@ -355,10 +355,11 @@ def main():
         if prec == "fp32" and tune_beam:
             ref_tune_recall = tune_recall(proj_d, link_d)
         if truth is not None:
-            r["recall_at_10"] = round(recall_at_10(
+            r["link_graph_recall_at_10"] = round(recall_at_10(
                 eval_ctx, eq_d, base_d, link_d, truth, p_eval.topk), 4)
             log(f"{prec}: recall@10 of the link graph "
-                f"({truth.shape[0]} held-out queries) = {r['recall_at_10']}")
+                f"({truth.shape[0]} held-out queries) = "
+                f"{r['link_graph_recall_at_10']}")
             r["final_index_recall_at_10"] = final_index_recall(
                 final_index(proj_d, link_d), base_d, eq_d, truth, n, dim)
             log(f"{prec}: recall@10 of the final index (reference "
@@ -443,9 +444,12 @@ def main():
         extra[f"speedup_{sp}_vs_fp32"] = round(
             results["fp32"]["ms_per_step"] / results[sp]["ms_per_step"], 3)
         if truth is not None:
-            extra[f"recall_delta_pt_{sp}"] = round(
-                100 * (results[sp]["recall_at_10"] -
-                       results["fp32"]["recall_at_10"]), 2)
+            # the index's recall (the tolerance of the metric) and the GPU
+            # stage's link graph alone (a proxy, searched by the build's own
+            # searcher at the workload's L)
+            extra[f"link_graph_recall_delta_pt_{sp}"] = round(
+                100 * (results[sp]["link_graph_recall_at_10"] -
+                       results["fp32"]["link_graph_recall_at_10"]), 2)
             extra[f"final_index_recall_delta_pt_{sp}"] = round(
                 100 * (results[sp]["final_index_recall_at_10"] -
                        results["fp32"]["final_index_recall_at_10"]), 2)
