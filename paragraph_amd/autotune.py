@@ -268,7 +268,7 @@ def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
         raise ValueError("the FP32 path keeps the reference beam mapping")
     if Ls is None:
         lo = max(48, p.M)
-        Ls = list(range(p.Km, lo - 1, -16))
+        Ls = list(range(p.Km, lo - 1, -8))
     n = p.n_base
     i32 = dict(dtype=torch.int32, device=base_d.device)
     proj = torch.empty((n, p.proj_deg), **i32)
