@@ -282,7 +282,7 @@ def main():
         tq_d, tq_truth = exact_top10(datagen.tune_queries(args.config,
                                                           args.n_tune))
 
-        # the screened beam is judged on the recall of the index it builds
+        # the screened beam is scored on the recall of the index it builds
         def tune_recall(proj, link):
             return final_index_recall(final_index(proj, link), base_d, tq_d,
                                       tq_truth, n, dim)
