@@ -3,7 +3,11 @@ ragged gt) through size-independent checks: the stage-by-stage pipeline
 through the C ABI, with the oracle re-running search, prune and merge on a
 random node sample from the same inputs (FP32 rows must match id-for-id);
 the whole-stage entry equal to the stage-by-stage composition; determinism;
-row invariants.
+row invariants.  The 1M x 200 build also runs at full size on the
+BF16-screened path with its tuned settings (beam 56, persistent
+locality-ordered queue, visited cache of 960 slots): its search is
+screened, so only the prune and merge, which decide every edge at FP32,
+are checked against the oracle on the sample.
 
 Default sizes are scaled down (1M and 100K base rows, exact gt).  With
 PG_SCALE_FULL=1 the configurations run at their full sizes; gt is then the
@@ -21,8 +25,12 @@ from paragraph_amd import _native, datagen
 pytestmark = pytest.mark.gpu
 TOMB = 0xFFFFFFFF
 FULL = os.environ.get("PG_SCALE_FULL") == "1"
-CASES = [("cfg4_10m_d200", 1.0 if FULL else 0.1),
-         ("cfg5_5m_d512_zipf", 1.0 if FULL else 0.02)]
+# (config, scale, precision, parameter overrides, beam)
+CASES = [("cfg4_10m_d200", 1.0 if FULL else 0.1, "fp32", {}, None),
+         ("cfg5_5m_d512_zipf", 1.0 if FULL else 0.02, "fp32", {}, None),
+         ("cfg2_1m_d200", 1.0, "bf16",
+          dict(precision=_native.PG_BF16, tune_flags=0x221, table_size=960),
+          56)]
 
 
 def _u(t):
@@ -36,17 +44,18 @@ def _sorted_segments(off, srcs):
     return srcs[np.lexsort((srcs, seg))]
 
 
-@pytest.mark.parametrize("cfg,scale", CASES)
-def test_scale_sampled_parity(cfg, scale, cuda):
+@pytest.mark.parametrize("cfg,scale,prec,over,beam", CASES,
+                         ids=[f"{c[0]}-{c[2]}" for c in CASES])
+def test_scale_sampled_parity(cfg, scale, prec, over, beam, cuda):
     import torch
     w = datagen.make_config(cfg, scale=scale, device="cuda",
                             gt_subset=200_000 if FULL else None)
     base = w["base"]
     n, dim = base.shape
-    M, L = w["M"], w["L"]
+    M, L = w["M"], beam or w["L"]
     gt = np.ascontiguousarray(w["gt"].view(np.uint32))
-    print(f"{cfg} n={n} dim={dim} M={M} L={L}", flush=True)
-    p = _native.make_params(n, dim, n, M, L)
+    print(f"{cfg} {prec} n={n} dim={dim} M={M} L={L}", flush=True)
+    p = _native.make_params(n, dim, n, M, L, **over)
     i32 = dict(dtype=torch.int32, device=cuda)
     base_d = torch.from_numpy(base).to(cuda)
     gt_d = torch.from_numpy(gt.view(np.int32)).to(cuda)
@@ -89,9 +98,10 @@ def test_scale_sampled_parity(cfg, scale, cuda):
             ctx.run_search(base_d, base_d, graph_d, res_d)
             ctx.sync()
             res = _u(res_d)
-            osr = cpu_ref.search(base[S], base, graph, p.Km, p.Kp, p.Kd,
-                                 p.topk)
-            assert np.array_equal(res[S], osr), f"search round {r + 1}"
+            if prec == "fp32":
+                osr = cpu_ref.search(base[S], base, graph, p.Km, p.Kp, p.Kd,
+                                     p.topk)
+                assert np.array_equal(res[S], osr), f"search round {r + 1}"
             pr_d = torch.full((n, p.M), -1, **i32)
             pc_d = torch.zeros(n, **i32)
             ctx.run_prune_rows(base_d, res_d, p.M, pr_d, pc_d)
