@@ -3,11 +3,13 @@ ragged gt) through size-independent checks: the stage-by-stage pipeline
 through the C ABI, with the oracle re-running search, prune and merge on a
 random node sample from the same inputs (FP32 rows must match id-for-id);
 the whole-stage entry equal to the stage-by-stage composition; determinism;
-row invariants.  The 1M x 200 build also runs at full size on the
-BF16-screened path with its tuned settings (beam 56, persistent
-locality-ordered queue, visited cache of 960 slots): its search is
-screened, so only the prune and merge, which decide every edge at FP32,
-are checked against the oracle on the sample.
+row invariants.  The 1M x 200 and 1M x 512 builds also run at full size
+with their tuned settings: FP32 at 1M x 512 (persistent locality-ordered
+queue, merge in the same order), and the BF16-screened path at both (beam
+56 and 64, visited caches of 960 and 1024 slots; at 1M x 512 the split
+screen and the candidate-major prune).  A screened search is not the
+FP32 one, so there only the match, prune, reverse CSR and merge, which
+decide every edge at FP32, are checked against the oracle on the sample.
 
 Default sizes are scaled down (1M and 100K base rows, exact gt).  With
 PG_SCALE_FULL=1 the configurations run at their full sizes; gt is then the
@@ -30,7 +32,11 @@ CASES = [("cfg4_10m_d200", 1.0 if FULL else 0.1, "fp32", {}, None),
          ("cfg5_5m_d512_zipf", 1.0 if FULL else 0.02, "fp32", {}, None),
          ("cfg2_1m_d200", 1.0, "bf16",
           dict(precision=_native.PG_BF16, tune_flags=0x221, table_size=960),
-          56)]
+          56),
+         ("cfg3_1m_d512", 1.0, "fp32", dict(tune_flags=0x624), None),
+         ("cfg3_1m_d512", 1.0, "bf16",
+          dict(precision=_native.PG_BF16, tune_flags=0x7A1, table_size=1024),
+          64)]
 
 
 def _u(t):
