@@ -5,9 +5,10 @@ random node sample from the same inputs (FP32 rows must match id-for-id);
 the whole-stage entry equal to the stage-by-stage composition; determinism;
 row invariants.  The 1M x 200 and 1M x 512 builds also run at full size
 with their tuned settings: FP32 at 1M x 512 (persistent locality-ordered
-queue, merge in the same order), and the BF16-screened path at both (beam
+queue, merge in the same order), the BF16-screened path at both (beam
 56 and 64, visited caches of 960 and 1024 slots; at 1M x 512 the split
-screen and the candidate-major prune).  A screened search is not the
+screen and the candidate-major prune), and the FP8-screened path at
+1M x 200 (beam 48, visited cache of 768 slots).  A screened search is not the
 FP32 one, so there only the match, prune, reverse CSR and merge, which
 decide every edge at FP32, are checked against the oracle on the sample.
 
@@ -33,6 +34,9 @@ CASES = [("cfg4_10m_d200", 1.0 if FULL else 0.1, "fp32", {}, None),
          ("cfg2_1m_d200", 1.0, "bf16",
           dict(precision=_native.PG_BF16, tune_flags=0x221, table_size=960),
           56),
+         ("cfg2_1m_d200", 1.0, "fp8",
+          dict(precision=_native.PG_FP8E4M3, tune_flags=0x221,
+               table_size=768), 48),
          ("cfg3_1m_d512", 1.0, "fp32", dict(tune_flags=0x624), None),
          ("cfg3_1m_d512", 1.0, "bf16",
           dict(precision=_native.PG_BF16, tune_flags=0x7A1, table_size=1024),
