@@ -2200,12 +2200,15 @@ static hipError_t launch_search_eval(
   for (uint32_t sh = 0; sh < 31; ++sh)
     if ((1u << sh) == Kd) kd_shift = sh;
   const uint32_t kpkd = Kp * Kd;
-  // top-list / candidate slices per lane: 2/2 up to 128 entries each, 3/3
-  // up to 192, else 4/4 (lists beyond 256 sort fully every iteration)
-  const int slices = (Km <= 128 && kpkd <= 128) ? 2
+  // top-list / candidate slices per lane: 1/1 up to 64 entries each (the
+  // narrow screened beams; fewer live registers, no spills), 2/2 up to 128,
+  // 3/3 up to 192, else 4/4 (lists beyond 256 sort fully every iteration)
+  const int slices = (Km <= 64 && kpkd <= 64) ? 1
+                     : (Km <= 128 && kpkd <= 128) ? 2
                      : (Km <= 192 && kpkd <= 192) ? 3 : 4;
   const void* fn =
-      slices == 2 ? reinterpret_cast<const void*>(&search_kernel<Eval, 2, 2>)
+      slices == 1 ? reinterpret_cast<const void*>(&search_kernel<Eval, 1, 1>)
+      : slices == 2 ? reinterpret_cast<const void*>(&search_kernel<Eval, 2, 2>)
       : slices == 3 ? reinterpret_cast<const void*>(&search_kernel<Eval, 3, 3>)
                     : reinterpret_cast<const void*>(&search_kernel<Eval, 4, 4>);
   (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2230,7 +2233,8 @@ static hipError_t launch_search_eval(
       ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter, \
       entry_seed, max_iters, table_size, lp2, kd_shift, out, stats, queue,  \
       queue ? order : nullptr)
-  if (slices == 2) PG_SEARCH_LAUNCH(2, 2);
+  if (slices == 1) PG_SEARCH_LAUNCH(1, 1);
+  else if (slices == 2) PG_SEARCH_LAUNCH(2, 2);
   else if (slices == 3) PG_SEARCH_LAUNCH(3, 3);
   else PG_SEARCH_LAUNCH(4, 4);
 #undef PG_SEARCH_LAUNCH
