@@ -1424,8 +1424,10 @@ struct EvalBF16 {
   static constexpr bool kExactVisited = false;
   // 6 waves per SIMD (<= 80 VGPRs, a few spilled) for one group of <= 256
   // padded elements: measured 3% faster than 5 waves on the 1M x 200
-  // search; 7 waves (72 VGPRs) spill more and gain nothing
-  static constexpr int kMinWaves = (NCH <= 8 && G == 1) ? 6 : 1;
+  // search; 7 waves (72 VGPRs) spill more and gain nothing.  Up to 512
+  // padded elements: 4 waves (<= 128 VGPRs, no spills; 3 untargeted)
+  static constexpr int kMinWaves =
+      (NCH <= 8 && G == 1) ? 6 : (NCH <= 16 && G == 1) ? 4 : 1;
   // The query's B-operand fragments live in LDS (qs[4c + k] = elements
   // 32c + 8k .. +7), read per MFMA: replicating them in registers would
   // cost 4*NCH VGPRs per lane.
