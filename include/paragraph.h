@@ -251,6 +251,24 @@ int pg_run_fusion_final(pg_ctx* ctx, const pg_host_params* hp,
 
 int pg_sync(pg_ctx* ctx);  /* hipStreamSynchronize + error check */
 
+/* Allocation-free variant of pg_run_search.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, serving loops, harnesses that forbid hipMalloc),
+ * so it takes no context.  It runs the exact FP32 search of *p
+ * (p->precision must be PG_FP32) on the current device, enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream), and calls no
+ * allocating routine: its work counters and locality order live in a
+ * fixed device array of the library (8 MiB + 2.3 KiB per GPU, held until
+ * the process ends).  Limit: with the locality order on (tune_flags bits 5
+ * and 9) n_query <= 2^20, else PG_ERR_INVALID.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU race: serialise them
+ * (one stream, or events).  Errors are reported by pg_last_error(NULL).
+ * Outputs equal pg_run_search's for the same params and inputs. */
+int pg_search_noalloc(const pg_params* p, const float* d_queries,
+                      uint32_t n_query, const float* d_base,
+                      const uint32_t* d_graph, uint32_t graph_deg,
+                      uint32_t* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

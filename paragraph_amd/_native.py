@@ -50,7 +50,7 @@ EXPORTED = (
     "pg_get_stats", "pg_reset_stats", "pg_get_stage_counters", "pg_last_error", "pg_stage_name",
     "pg_run_match", "pg_run_prune", "pg_run_reverse", "pg_run_merge",
     "pg_run_search", "pg_run_pair_distances", "pg_sync", "pg_run_fusion",
-    "pg_run_bipartite_async", "pg_run_fusion_final",
+    "pg_run_bipartite_async", "pg_run_fusion_final", "pg_search_noalloc",
 )
 
 _lib = None
@@ -105,6 +105,8 @@ def _declare(L):
     L.pg_run_fusion_final.argtypes = [_P, _P, _P, _P, _U32, _P, _U32, _P, _P]
     L.pg_run_pair_distances.argtypes = [_P, _P, _U32, _P, _P, _U32,
                                         ctypes.c_int, _P]
+    L.pg_search_noalloc.argtypes = [ctypes.POINTER(PgParams), _P, _U32, _P,
+                                    _P, _U32, _P, _P]
 
 
 def stage_name(i):
@@ -122,6 +124,19 @@ def make_params(n_base, dim, n_query, M, L, **overrides):
             raise KeyError(k)
         setattr(p, k, int(v))
     return p
+
+
+def search_noalloc(params, queries, base, graph, out):
+    """pg_search_noalloc: the exact FP32 search without a context, on
+    torch's current stream of the tensors' device (no device allocation;
+    its scratch is a fixed array of the library)."""
+    import torch
+    stream = torch.cuda.current_stream(queries.device).cuda_stream
+    rc = lib().pg_search_noalloc(
+        ctypes.byref(params), _dptr(queries), queries.shape[0], _dptr(base),
+        _dptr(graph), graph.shape[1], _dptr(out), ctypes.c_void_p(stream))
+    if rc != 0:
+        raise PgError(f"rc={rc}: {lib().pg_last_error(None).decode()}")
 
 
 def check_params(p):

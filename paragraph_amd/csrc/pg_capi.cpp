@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "../../include/paragraph.h"
@@ -952,6 +953,119 @@ int pg_run_fusion(pg_ctx* c, const pg_host_params* hp, const float* d_base,
                               d_off, d_ids, c->stream));
   c->bip_valid = true;
   c->bip_hp = *hp;
+  return PG_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free search (pg_search_noalloc).  The reference has no
+// counterpart; it serves callers that cannot allocate device memory (code
+// under hipGraph capture, serving loops).  Its device scratch is one
+// file-scope array per GPU, held until the process ends:
+//   words [0, 64)                     persistent-search queue counters
+//   words [64, 64 + 2*kOrderGroups)   locality-order group counters
+//   then kNoAllocMaxQueries words     pivot group of each query
+//   then kNoAllocMaxQueries words     the locality order
+// Host code never uses the array's name as an address: the device address
+// comes from hipGetSymbolAddress, once per device, on the first call made
+// with that device current.  Every call on one GPU shares the array, so
+// calls that overlap on one GPU race; every region a call reads is
+// rewritten or cleared (by fill kernels, no memset) inside that call.
+namespace {
+
+constexpr uint32_t kNoAllocMaxQueries = 1u << 20;
+constexpr size_t kScratchQueueWords = 64;
+constexpr size_t kScratchCntWords = 2ull * pg::kOrderGroups;
+constexpr size_t kScratchWords = kScratchQueueWords + kScratchCntWords +
+                                 2ull * kNoAllocMaxQueries;
+static_assert(pg::kQueueSegs <= kScratchQueueWords, "queue counters fit");
+
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once).
+int scratch_address(int dev, uint32_t** out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_err(nullptr, PG_ERR_INVALID,
+                   "pg_search_noalloc: device %d beyond the %d supported", dev,
+                   kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+      return set_err(nullptr, PG_ERR_HIP,
+                     "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                     hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<uint32_t*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pg_search_noalloc(const pg_params* p, const float* d_queries,
+                      uint32_t n_query, const float* d_base,
+                      const uint32_t* d_graph, uint32_t graph_deg,
+                      uint32_t* d_out, void* stream) {
+  char msg[256];
+  if (pg_params_check(p, msg, sizeof(msg)) != PG_OK)
+    return set_err(nullptr, PG_ERR_INVALID, "invalid params: %s", msg);
+  if (!d_base || !d_graph || !d_out || (n_query && !d_queries))
+    return set_err(nullptr, PG_ERR_INVALID, "NULL argument");
+  if (p->precision != PG_FP32)
+    return set_err(nullptr, PG_ERR_INVALID,
+                   "pg_search_noalloc runs the exact FP32 search only (the "
+                   "screened searches read a per-build copy of the base)");
+  if (p->Kd > graph_deg)
+    return set_err(nullptr, PG_ERR_INVALID, "Kd (%u) exceeds graph degree (%u)",
+                   p->Kd, graph_deg);
+  const bool persistent = (p->tune_flags & kTunePersistent) != 0;
+  const bool ordered = persistent && (p->tune_flags & kTuneOrder) != 0;
+  if (ordered && n_query > kNoAllocMaxQueries)
+    return set_err(nullptr, PG_ERR_INVALID,
+                   "pg_search_noalloc: n_query %u exceeds the %u queries its "
+                   "scratch holds for the locality order (clear tune_flags "
+                   "bit 9 for larger batches)",
+                   n_query, kNoAllocMaxQueries);
+  if (n_query == 0) return PG_OK;
+  int dev = 0;
+  {
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+      return set_err(nullptr, PG_ERR_NO_DEVICE, "hipGetDevice: %s",
+                     hipGetErrorString(e));
+  }
+  uint32_t* ws = nullptr;
+  PG_TRY(scratch_address(dev, &ws));
+  uint32_t* queue = ws;
+  uint32_t* cnt = ws + kScratchQueueWords;
+  uint32_t* group = cnt + kScratchCntWords;
+  uint32_t* order = group + kNoAllocMaxQueries;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (ordered) {
+    const hipError_t e = pg::launch_locality_order(d_queries, n_query, p->dim,
+                                                   group, cnt, order, s);
+    if (e != hipSuccess)
+      return set_err(nullptr, PG_ERR_HIP, "locality order: %s",
+                     hipGetErrorString(e));
+  }
+  const hipError_t e = pg::launch_search(
+      d_queries, n_query, d_base, nullptr, nullptr, nullptr, 0, p->n_base,
+      p->dim, d_graph, graph_deg, p->Km, p->Kp, p->Kd, p->topk, p->reset_iter,
+      p->entry_seed, p->max_iters, effective_table_size(*p), tune_gather(*p),
+      d_out, nullptr, persistent ? queue : nullptr, ordered ? order : nullptr,
+      s);
+  if (e != hipSuccess)
+    return set_err(nullptr, PG_ERR_HIP, "search launch: %s",
+                   hipGetErrorString(e));
   return PG_OK;
 }
 
