@@ -48,6 +48,29 @@ def test_hand_search(hand):
     assert out.tolist() == c["rows"]
 
 
+SEARCH_HAND = [("search_reset15", r) for r in ("15", "14", "16", "1000")] \
+    + [("search_collect_kp2", "15")]
+
+
+@pytest.mark.parametrize("name,reset", SEARCH_HAND,
+                         ids=[f"{n}-reset{r}" for n, r in SEARCH_HAND])
+def test_hand_search_traces(hand, name, reset):
+    """Hand-derived K4 traces: the reset period of the visited table, the
+    collect order with Kp = 2, and the FLT_MAX tail order (topk = Km +
+    Kp*Kd shows the whole list)."""
+    c = hand[name]
+    st = cpu_ref.OrcStats()
+    out = cpu_ref.search(np.array(c["queries"], np.float32),
+                         np.array(c["base"], np.float32),
+                         np.array(c["graph"], np.uint32), c["Km"], c["Kp"],
+                         c["Kd"], c["topk"], reset_iter=int(reset),
+                         entry_seed=c["entry_seed"], stats=st)
+    assert out.tolist() == c["rows_by_reset_iter"][reset]
+    if reset == "15":
+        assert st.search_iters == c["search_iters"]
+        assert st.search_dist == c["search_dist"]
+
+
 @pytest.mark.parametrize("name", golden_io.stage_names())
 def test_stage_regression(name):
     g = golden_io.load_stage(name)

@@ -97,6 +97,40 @@ def test_hand_search(hand, cuda):
     assert _host(out).tolist() == c["rows"]
 
 
+SEARCH_HAND = [("search_reset15", r) for r in ("15", "14", "16", "1000")] \
+    + [("search_collect_kp2", "15")]
+
+
+@pytest.mark.parametrize("name,reset", SEARCH_HAND,
+                         ids=[f"{n}-reset{r}" for n, r in SEARCH_HAND])
+@pytest.mark.parametrize("flags", [0x0, 0x20], ids=["grid", "persistent"])
+def test_hand_search_traces(hand, name, reset, flags, cuda):
+    """The hand-derived K4 traces (reset period, Kp = 2 collect order, the
+    FLT_MAX tail) through pg_run_search, rows and counters."""
+    import torch
+    c = hand[name]
+    base = np.array(c["base"], np.float32)
+    q = np.array(c["queries"], np.float32)
+    graph = np.array(c["graph"], np.uint32)
+    n = base.shape[0]
+    p = _native.make_params(n, base.shape[1], n, 2, c["Km"], Kp=c["Kp"],
+                            Kd=c["Kd"], topk=c["topk"],
+                            proj_deg=graph.shape[1], rounds=1,
+                            reset_iter=int(reset), tune_flags=flags,
+                            entry_seed=c["entry_seed"])
+    assert _native.check_params(p)[0] == 0, _native.check_params(p)
+    out = torch.empty((q.shape[0], c["topk"]), dtype=torch.int32, device=cuda)
+    with _native.Context(p) as ctx:
+        ctx.run_search(_dev(q, torch), _dev(base, torch), _dev(graph, torch),
+                       out)
+        ctx.sync()
+        st = ctx.stats()
+    assert _host(out).tolist() == c["rows_by_reset_iter"][reset]
+    if reset == "15":
+        assert st.search_iters == c["search_iters"]
+        assert st.search_dist == c["search_dist"]
+
+
 @pytest.mark.parametrize("name", golden_io.stage_names())
 def test_stage_golden(name, cuda):
     import torch

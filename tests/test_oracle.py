@@ -8,7 +8,9 @@ oracle function is checked against what the mathematics fixes:
   - match / top-1 rows / reverse lists: closed forms and brute force;
   - RNG prune / merge: hand-derived 2-D cases and an independent brute-force
     restatement in Python with FP64 distances on well-separated points;
-  - search: a complete graph, where the result must be the exact top-Km.
+  - search: a complete graph, where the result must be the exact top-Km;
+    hand-derived traces (tests/golden: the reset period, the Kp = 2 collect
+    order, the FLT_MAX tail); the termination fixed point on random graphs.
 """
 from fractions import Fraction
 
@@ -291,6 +293,68 @@ def test_merge_cap_keeps_smallest_sources():
     assert st.rev_overflow == 1 and st.merge_dist == 3 + 9
 
 
+def merge_fp64(base, b, rev_srcs, cap, pruned, deg, strict):
+    """Independent restatement of merge_and_sort + the merge prune
+    (Gbuilder.cuh:1433-1601) in FP64: candidates in slot order (the cap
+    smallest reverse sources ascending, then the pruned list), stable sort
+    by distance, list[0] kept, strict RNG (occluded when a pair distance to
+    a kept entry is strictly below the candidate's) up to deg, and the
+    non-strict fill: mark the first list position of each kept id, append
+    the unmarked entries in list order.  Exact on small-integer data, where
+    FP32 and FP64 give the same values (ties included)."""
+    X = base.astype(np.float64)
+    n = len(base)
+    cand = sorted(rev_srcs)[:cap] + list(pruned)
+    if not cand:
+        return [TOMB] * deg
+    d = [-(X[b] @ X[c]) for c in cand]
+    order = sorted(range(len(cand)), key=lambda j: (d[j], j))
+    lst = [cand[j] for j in order]
+    dl = [d[j] for j in order]
+    kept = [lst[0]]
+    for j in range(1, len(lst)):
+        if len(kept) == deg:
+            break
+        c = lst[j]
+        if not any(-(X[c] @ X[k]) < dl[j] for k in kept):
+            kept.append(c)
+    if not strict and len(kept) < deg:
+        marked = [False] * len(lst)
+        for k in kept:
+            marked[lst.index(k)] = True
+        for j, c in enumerate(lst):
+            if len(kept) >= deg:
+                break
+            if not marked[j] and c != TOMB and c != b and c < n:
+                kept.append(c)
+    return kept + [TOMB] * (deg - len(kept))
+
+
+@pytest.mark.parametrize("strict", [True, False])
+def test_merge_brute_force_integer_data(strict):
+    """The oracle's merge against merge_fp64 on small-integer points (exact
+    distances, so equal distances occur and the stable order decides),
+    with capped reverse lists and mutual edges (an id in both lists)."""
+    rng = np.random.default_rng(12)
+    for trial in range(40):
+        n, dim = 14, 3
+        base = rng.integers(-3, 4, (n, dim)).astype(np.float32)
+        mp = int(rng.integers(2, 6))
+        cap = int(rng.integers(1, 8))
+        deg = int(rng.integers(2, 9))
+        pr = np.full((n, mp), TOMB, np.uint32)
+        pc = rng.integers(0, mp + 1, n).astype(np.uint32)
+        for b in range(n):
+            others = [u for u in range(n) if u != b]
+            pr[b, :pc[b]] = rng.choice(others, size=pc[b], replace=False)
+        off, srcs = cpu_ref.reverse_csr(pr, pc)
+        out = cpu_ref.merge(base, off, srcs, cap, pr, pc, deg, strict)
+        for b in range(n):
+            exp = merge_fp64(base, b, srcs[off[b]:off[b + 1]].tolist(), cap,
+                             pr[b, :pc[b]].tolist(), deg, strict)
+            assert out[b].tolist() == exp, (trial, b)
+
+
 # -------------------------------------------------------------- search
 
 def complete_graph(n, deg):
@@ -329,6 +393,37 @@ def test_search_output_sorted_and_masked():
         assert (row != INT_MAX).all()
         d = [cpu_ref.search_dist(base[q], base[i]) for i in row]
         assert all(d[i] <= d[i + 1] for i in range(Km - 1))
+
+
+def test_search_termination_closure():
+    """At termination every top-Km entry is expanded, so every first-Kd
+    neighbour of a top-Km node is either in the top Km or sorts after its
+    last entry (key = (distance, id)): the search's fixed point, checked
+    with the oracle's own pinned distance on random graphs, with resets."""
+    rng = np.random.default_rng(13)
+    for trial in range(4):
+        n, dim = 400, 6
+        base = rng.standard_normal((n, dim)).astype(np.float32)
+        deg = 12
+        g = rng.integers(0, n, (n, deg)).astype(np.uint32)
+        g[rng.random((n, deg)) < 0.15] = TOMB
+        Km, Kp, Kd = 24, 2, 12
+        qs = rng.standard_normal((20, dim)).astype(np.float32)
+        st = cpu_ref.OrcStats()
+        out = cpu_ref.search(qs, base, g, Km, Kp, Kd, Km + Kp * Kd,
+                             reset_iter=3, stats=st)
+        assert st.search_capped == 0
+        for q, row in zip(qs, out):
+            top = [int(v) for v in row[:Km]]
+            real = [v for v in top if v != INT_MAX]
+            key = lambda v: (cpu_ref.search_dist(q, base[v]), v)
+            last = key(real[-1]) if len(real) == Km else None
+            for u in real:
+                for v in g[u, :Kd]:
+                    v = int(v)
+                    if v == TOMB or v in top:
+                        continue
+                    assert last is not None and key(v) > last, (trial, u, v)
 
 
 def test_search_stats_and_watchdog():
