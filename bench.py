@@ -24,31 +24,40 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz; a wave64 VALU instruction occupies
+# its SIMD for 4 cycles (16 lanes per cycle), so the issue peak is
+# 1024 x 2.4e9 / 4 wave instructions per second.
+N_SIMDS = 1024
+CLOCK_HZ = 2.4e9
+VALU_PEAK_GIPS = N_SIMDS * CLOCK_HZ / 4 / 1e9
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(kernel, workload, precision):
-    """HBM bytes per launch recorded from rocprofv3 FETCH_SIZE/WRITE_SIZE
-    runs (scaled by a 1 GiB copy) in profiles/*.md, or None."""
+def profiled(name, kernel, workload, precision):
+    """A per-launch figure `name[kernel,workload,precision] = value`
+    recorded in profiles/*.md from rocprofv3 runs (HBM bytes scaled by a
+    1 GiB copy, VALU instructions, MFMA busy cycles, mean launch ms), or
+    None."""
     import glob
     import re
-    key = f"traffic_bytes_per_launch[{kernel},{workload},{precision}]"
+    key = f"{name}[{kernel},{workload},{precision}]"
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
             if line.startswith(key):
-                m = re.search(r"=\s*([0-9]+)", line)
+                m = re.search(r"=\s*([0-9.eE+-]+)", line)
                 if m:
-                    return int(m.group(1))
+                    return float(m.group(1))
     return None
 
 
 def stage_bytes(kind, c, p, n, D=None, mp=None, deg=None, search_row=None):
-    """Algorithmic HBM bytes of one launch from its counters.  FP32 rows for
-    prune/merge; the search reads FP32 rows, or padded BF16 rows in the
-    screened path (search_row bytes)."""
+    """Algorithmic bytes of one launch from its counters: each distinct
+    (node-task, row) pair once, rows charged at s * dim bytes (no padding).
+    FP32 rows for prune/merge; the search reads FP32 rows, or the screened
+    copy's rows (search_row bytes)."""
     row = 4 * p.dim
     if kind == "search":
         srow = search_row or row
@@ -79,17 +88,22 @@ def time_build(ctx, base_d, gt_d, proj_d, link_d, steps, warmup, label):
 
 
 def roofline_of(ctx, p, n, workload, precision):
-    """Dominant kernel of the last build: algorithmic bytes per launch over
-    its mean launch time (HIP events on the build stream)."""
+    """Dominant kernel of the last build against the resource that binds
+    it.  Three fractions over its mean launch time (HIP events on the build
+    stream): HBM (rocprof FETCH_SIZE + WRITE_SIZE bytes per launch, scaled
+    by a 1 GiB copy, over the chip's 8 TB/s), VALU issue (rocprof
+    SQ_INSTS_VALU per launch over 1024 SIMDs x 2.4 GHz / 4 cycles) and MFMA
+    (SQ_VALU_MFMA_BUSY_CYCLES per launch over 1024 SIMDs x 2.4 GHz); the
+    largest is `bound`.  The algorithmic bytes (rows at s * dim bytes,
+    cache-inclusive) over the same time are reported beside them."""
     from paragraph_amd import _native
     st = ctx.stats()
     stage_ms = [float(st.stage_ms[i]) for i in range(st.n_stages)]
     names = [_native.stage_name(i) for i in range(st.n_stages)]
     counters = [ctx.stage_counters(i) for i in range(st.n_stages)]
-    dpad = (p.dim + 31) // 32 * 32
-    # bytes of one base row as the search reads it: padded BF16, padded
+    # bytes of one base row as the search reads it (no padding): BF16,
     # e4m3 plus its FP32 scale, or FP32
-    row = {"bf16": 2 * dpad, "fp8": dpad + 4}.get(precision, 4 * p.dim)
+    row = {"bf16": 2 * p.dim, "fp8": p.dim + 4}.get(precision, 4 * p.dim)
     kinds = {}
     for i, nm in enumerate(names):
         kind = nm.split("_")[0]
@@ -100,6 +114,10 @@ def roofline_of(ctx, p, n, workload, precision):
         deg = p.proj_deg if nm == "merge_proj" else p.M
         b = stage_bytes(kind, counters[i], p, n, D=D, mp=mp, deg=deg,
                         search_row=row)
+        # link rounds only: the projection launches are small and of
+        # another shape
+        if nm.endswith("_proj"):
+            continue
         k = kinds.setdefault(kind, {"ms": 0.0, "bytes": 0, "launches": 0})
         k["ms"] += stage_ms[i]
         k["bytes"] += b
@@ -107,27 +125,46 @@ def roofline_of(ctx, p, n, workload, precision):
     dom = max(kinds, key=lambda k: kinds[k]["ms"])
     kd = kinds[dom]
     avg_ms = kd["ms"] / kd["launches"]
+    avg_s = avg_ms / 1000.0
     bytes_per_launch = kd["bytes"] / kd["launches"]
-    achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9
+    alg_gbs = bytes_per_launch / avg_s / 1e9
     # the link-round prune runs as the kept-major sweep unless tune_flags
     # bit 8 selects the candidate-major kernel
     prune_k = "prune_kernel" if p.tune_flags & 0x100 else "prune_sweep_kernel"
     kname = {"search": "search_kernel", "prune": prune_k,
              "merge": "merge_kernel", "reverse": "reverse_*"}.get(dom, dom)
-    traffic = profiled_traffic(kname, workload, precision)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic,
-                "kernel": kname,
-                "avg_launch_ms": round(avg_ms, 3),
-                "alg_bytes_per_launch": int(bytes_per_launch)}
+    prof = lambda what: profiled(what, kname, workload, precision)
+    traffic = prof("traffic_bytes_per_launch")
+    valu = prof("valu_insts_per_launch")
+    mfma = prof("mfma_busy_cycles_per_launch")
+    fr = {}
     if traffic:
-        # measured HBM rate: the profiled bytes per launch over the same
-        # launch time (algorithmic bytes above it come from L2 / MALL)
-        hbm = traffic / (avg_ms / 1000.0) / 1e9
-        roofline["hbm_gbs"] = round(hbm, 1)
-        roofline["hbm_frac"] = round(hbm / HBM_PEAK_GBS, 4)
+        fr["hbm"] = (traffic / avg_s / 1e9, HBM_PEAK_GBS, "GB/s")
+    if valu:
+        fr["valu"] = (valu / avg_s / 1e9, VALU_PEAK_GIPS,
+                      "G wave64 VALU instructions/s")
+    if mfma:
+        fr["mfma"] = (100.0 * mfma / (N_SIMDS * CLOCK_HZ * avg_s), 100.0,
+                      "% of MFMA cycles busy")
+    if fr:
+        bound = max(fr, key=lambda k: fr[k][0] / fr[k][1])
+    else:  # no profile for this kernel: the algorithmic byte rate only
+        bound, fr["hbm"] = "hbm", (alg_gbs, HBM_PEAK_GBS, "GB/s")
+    a, pk, unit = fr[bound]
+    roofline = {"bound": bound, "achieved": round(a, 2), "peak": pk,
+                "unit": unit, "frac": round(a / pk, 4),
+                "traffic": int(traffic) if traffic else None,
+                "kernel": kname, "avg_launch_ms": round(avg_ms, 3),
+                "profile_avg_launch_ms": prof("profile_avg_launch_ms"),
+                "alg_bytes_per_launch": int(bytes_per_launch),
+                "alg_gbs": round(alg_gbs, 1),
+                "alg_frac": round(alg_gbs / HBM_PEAK_GBS, 4),
+                "fractions": {k: round(v[0] / v[1], 4)
+                              for k, v in fr.items()}}
+    if traffic:
+        roofline["hbm_gbs"] = round(traffic / avg_s / 1e9, 1)
+        roofline["hbm_frac"] = round(traffic / avg_s / 1e9 / HBM_PEAK_GBS,
+                                     4)
     detail = {"stage_ms": {nm: round(x, 3) for nm, x in zip(names, stage_ms)},
               "stage_counters": {nm: c for nm, c in zip(names, counters)}}
     return roofline, detail
@@ -179,9 +216,13 @@ def main():
                     help="beam width L of the screened builds (Km = L, "
                          "Kp = ceil(L/16), topk = 2L), skipping the beam "
                          "tuner")
-    ap.add_argument("--n-tune", type=int, default=10000,
+    ap.add_argument("--n-tune", type=int, default=50000,
                     help="tuning queries the beam autotuner scores recall "
                          "on (a seed disjoint from the eval queries)")
+    ap.add_argument("--no-controls", action="store_true",
+                    help="skip the control builds (FP32 at the tuned "
+                         "beam, the screened path at the workload's beam, "
+                         "the final index without link rows)")
     ap.add_argument("--gt-subset", type=int, default=None,
                     help="gt = exact top-128 within shards of this many base "
                          "rows (default: exact over all rows up to 3M base "
@@ -282,11 +323,13 @@ def main():
         tq_d, tq_truth = exact_top10(datagen.tune_queries(args.config,
                                                           args.n_tune))
 
-        # the screened beam is scored on the recall of the index it builds
-        def tune_recall(proj, link):
-            return final_index_recall(final_index(proj, link), base_d, tq_d,
-                                      tq_truth, n, dim)
-    ref_tune_recall = None
+        # the screened beam is scored on the per-query recall of the index
+        # it builds
+        def tune_hits(proj, link):
+            return final_index_hits(final_index(proj, link), base_d, tq_d,
+                                    tq_truth, n, dim)
+    ref_tune_hits = None
+    ref_link = None
     if tune_beam and "fp32" not in precs:
         # the FP32 build's recall on the tuning queries is the reference
         with _native.Context(p32) as c32:
@@ -295,8 +338,9 @@ def main():
             lk_ = torch.empty((n, M), dtype=torch.int32, device=dev)
             c32.stage_device(base_d, gt_d, pr_, lk_)
             c32.wait()
-            ref_tune_recall = tune_recall(pr_, lk_)
-            del pr_, lk_
+            ref_tune_hits = tune_hits(pr_, lk_)
+            ref_link = lk_
+            del pr_
     # FP32 first: its recall anchors the beam tuning of the screened builds
     order = (["fp32"] if "fp32" in precs else []) + screened_req
     for prec in order:
@@ -317,8 +361,9 @@ def main():
         if args.autotune:
             tuned = {}
             if prec != "fp32" and tune_beam:
-                p, brep = autotune.tune_beam(base_d, gt_d, p, tune_recall,
-                                             ref_tune_recall, log=log)
+                p, brep = autotune.tune_beam(base_d, gt_d, p, tune_hits,
+                                             ref_tune_hits, log=log,
+                                             ref_link=ref_link)
                 tuned["beam"] = brep
                 log(f"{prec}: autotuned beam L={brep['L']} in "
                     f"{brep['seconds']} s")
@@ -353,7 +398,8 @@ def main():
         log(f"{prec}: {ms:.1f} ms/step, {r['value']:.0f} pts/s; stage ms "
             f"{ {k: round(v, 1) for k, v in detail['stage_ms'].items()} }")
         if prec == "fp32" and tune_beam:
-            ref_tune_recall = tune_recall(proj_d, link_d)
+            ref_tune_hits = tune_hits(proj_d, link_d)
+            ref_link = link_d
         if truth is not None:
             r["link_graph_recall_at_10"] = round(recall_at_10(
                 eval_ctx, eq_d, base_d, link_d, truth, p_eval.topk), 4)
@@ -372,8 +418,17 @@ def main():
         r["beam"] = {"L": int(p.Km), "Kp": int(p.Kp), "Kd": int(p.Kd),
                      "topk": int(p.topk)}
         r["link"] = link_d
+        r["proj"] = proj_d
         r["ctx"] = ctx
         results[prec] = r
+
+    # ---- controls: where the screened speed-up comes from
+    controls = None
+    if (not args.no_controls and "fp32" in results and truth is not None
+            and any(x != "fp32" for x in results)):
+        controls = beam_controls(results, p32, base_d, gt_d, n, M, L, dim,
+                                 args.steps, eval_ctx, p_eval, eq_d, truth,
+                                 final_index, log)
 
     # ---- host window: one build of the value path from pageable host
     # buffers through the host entry point (H2D + stage + D2H); reported
@@ -431,13 +486,16 @@ def main():
     extra = {}
     for prec, r in results.items():
         extra[prec] = {k: (round(v, 3) if isinstance(v, float) else v)
-                       for k, v in r.items() if k not in ("link", "ctx")}
+                       for k, v in r.items()
+                       if k not in ("link", "proj", "ctx")}
     if dist_err is not None:
         extra["distance_error_vs_fp64"] = dist_err
     if host_window is not None:
         extra["host_window"] = host_window
     if end_to_end is not None:
         extra["end_to_end"] = end_to_end
+    if controls is not None:
+        extra["controls"] = controls
     for sp in screened:
         if "fp32" not in results:
             break
@@ -502,6 +560,68 @@ def main():
     print(json.dumps(line), flush=True)
     for r in results.values():
         r["ctx"].close()
+
+
+def beam_controls(results, p32, base_d, gt_d, n, M, L, dim, steps,
+                  eval_ctx, p_eval, eq_d, truth, final_index, log):
+    """Control builds beside the value path, to separate what precision and
+    what the beam give: the FP32 path at the screened path's tuned beam,
+    the screened path at the workload's beam L (the precision-only
+    speed-up), and the final index of the FP32 build with its link rows
+    (and its projection rows) replaced by tombs, i.e. how much of the final
+    index's recall the GPU stage's rows carry."""
+    import torch
+    from paragraph_amd import _native, autotune
+    head = next(x for x in results if x != "fp32")
+    hp = results[head]["ctx"].params
+    f32 = results["fp32"]
+    i32 = dict(dtype=torch.int32, device=base_d.device)
+    out = {"screened_path": head}
+
+    def build(p, label):
+        proj = torch.empty((n, p.proj_deg), **i32)
+        link = torch.empty((n, M), **i32)
+        with _native.Context(p) as ctx:
+            ms = time_build(ctx, base_d, gt_d, proj, link, steps, 1, label)
+        r = {"L": int(p.Km), "ms_per_step": round(ms, 2),
+             "value": round(n / (ms / 1000.0), 1),
+             "link_graph_recall_at_10": round(recall_at_10(
+                 eval_ctx, eq_d, base_d, link, truth, p_eval.topk), 4),
+             "final_index_recall_at_10": final_index_recall(
+                 final_index(proj, link), base_d, eq_d, truth, n, dim),
+             "link_overlap_vs_fp32": autotune.link_overlap(f32["link"],
+                                                           link)}
+        log(f"control {label}: {r}")
+        return r
+
+    Lt = int(hp.Km)
+    if Lt != L:
+        out["fp32_at_tuned_beam"] = build(autotune.beam_params(p32, Lt),
+                                          f"fp32 L={Lt}")
+        pl = autotune.beam_params(hp, L)
+        out[f"{head}_at_workload_beam"] = build(pl, f"{head} L={L}")
+        out["speedup_precision_only"] = round(
+            f32["ms_per_step"] / out[f"{head}_at_workload_beam"]
+            ["ms_per_step"], 3)
+        out["speedup_same_beam_tuned"] = round(
+            out["fp32_at_tuned_beam"]["ms_per_step"] /
+            results[head]["ms_per_step"], 3)
+    out["speedup_beam_tuned"] = round(
+        f32["ms_per_step"] / results[head]["ms_per_step"], 3)
+    # the GPU stage's share of the final index: FusionFinal without the
+    # link rows, and without the link and projection rows
+    proj32 = results["fp32"]["proj"]
+    tomb_link = torch.full((n, M), -1, **i32)
+    tomb_proj = torch.full_like(proj32, -1)
+    out["fp32_final_index_recall_at_10"] = \
+        results["fp32"]["final_index_recall_at_10"]
+    out["fp32_final_index_recall_without_link_rows"] = final_index_recall(
+        final_index(proj32, tomb_link), base_d, eq_d, truth, n, dim)
+    out["fp32_final_index_recall_without_link_or_projection_rows"] = \
+        final_index_recall(final_index(tomb_proj, tomb_link), base_d, eq_d,
+                           truth, n, dim)
+    log(f"controls: {out}")
+    return out
 
 
 def time_end_to_end(results, precs, w, gt_u32, base_d, gt_d, n, M, log):
@@ -591,10 +711,13 @@ def time_end_to_end(results, precs, w, gt_u32, base_d, gt_d, n, M, log):
             "_final": (off, ids)}
 
 
-def final_index_recall(final, base_d, eq_d, truth, n, dim):
-    """recall@10 of the final index with the reference's evaluation
-    searcher knobs (search_shared_hashtable, Gsearcher.cuh:108-296: Km 128,
-    Kp 6, Kd 64, reset every 4) over its lists padded to rows."""
+def final_index_hits(final, base_d, eq_d, truth, n, dim):
+    """Per-query recall@10 hits (0..10) of the final index with the
+    reference's evaluation searcher knobs (search_shared_hashtable,
+    Gsearcher.cuh:108-296: Km 128, Kp 6, Kd 64, reset every 4) over its
+    lists padded to rows.  Unlike the reference's searcher, each query
+    starts from an empty visited table (the reference's warp keeps the
+    table of its previous query, Gsearcher.cuh:206-210)."""
     import torch
     from paragraph_amd import _native, io
     off, ids = final
@@ -608,8 +731,14 @@ def final_index_recall(final, base_d, eq_d, truth, n, dim):
         ctx.run_search(eq_d, base_d, rows_d, out_d)
         ctx.sync()
     found = out_d[:, :10].cpu().numpy()
-    hits = sum(len(set(a) & set(b)) for a, b in zip(found, truth))
-    return round(hits / truth.size, 4)
+    # the search emits distinct ids, so hits = matches against the truth row
+    return (found[:, :, None] == truth[:, None, :10]).any(axis=2).sum(axis=1)
+
+
+def final_index_recall(final, base_d, eq_d, truth, n, dim):
+    """recall@10 of the final index (final_index_hits over all queries)."""
+    return round(float(final_index_hits(final, base_d, eq_d, truth, n,
+                                        dim).sum()) / truth.size, 4)
 
 
 def distance_error(ctxs, w, base_d, n_sample=2000):

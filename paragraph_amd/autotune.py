@@ -250,19 +250,40 @@ def beam_params(p, L):
     return pp
 
 
-def recall_floor(ref_recall, tol_pt=0.5, margin_pt=0.2, rel_tol=0.05):
-    """Lowest recall@10 the beam tuner accepts for a screened build."""
-    return ref_recall - min((tol_pt - margin_pt) / 100.0,
-                            rel_tol * ref_recall)
+def beam_accepts(hits, ref_hits, tol_pt=0.5, z=2.0, rel_tol=0.05):
+    """Acceptance of a screened beam from per-query recall@10 hits (0..10
+    per tuning query) against the FP32 build's on the same queries.  The
+    paired per-query difference gives the mean recall change and its
+    standard error; the beam passes when the lower confidence bound
+    (mean - z * se, z = 2: about 97.7% one-sided) of the change stays above
+    -min(tol_pt / 100, rel_tol * ref_recall): the tolerance in points,
+    capped at a relative loss for workloads whose recall is itself small.
+    Returns (ok, delta, lower_bound, tolerance), recall units."""
+    import numpy as np
+    h = np.asarray(hits, dtype=np.float64) / 10.0
+    r = np.asarray(ref_hits, dtype=np.float64) / 10.0
+    if h.shape != r.shape or h.size == 0:
+        raise ValueError("hits and ref_hits must be per-query arrays of one "
+                         "shape")
+    d = h - r
+    delta = float(d.mean())
+    se = float(d.std(ddof=1) / np.sqrt(d.size)) if d.size > 1 else 0.0
+    tol = min(tol_pt / 100.0, rel_tol * float(r.mean()))
+    lower = delta - z * se
+    return lower >= -tol, delta, lower, tol
 
 
-def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
-              margin_pt=0.2, rel_tol=0.05, reps=2, log=None):
-    """Screened builds: the fastest beam width L whose recall@10
-    (recall_fn(proj_d, link_d), scored on tuning queries) is at least
-    ref_recall - min((tol_pt - margin_pt) / 100, rel_tol * ref_recall): an
-    absolute tolerance in points, capped at a relative loss for workloads
-    whose recall is itself small.  Returns (params, report)."""
+def tune_beam(base_d, gt_d, p, hits_fn, ref_hits, Ls=None, tol_pt=0.5,
+              z=2.0, rel_tol=0.05, reps=2, log=None, ref_link=None):
+    """Screened builds: the beam width L (Km = L, Kp = ceil(L/16), topk =
+    2L) of the build, walked down from the workload's L in steps of 8 to
+    max(48, M) and stopped at the first beam that fails beam_accepts
+    (hits_fn(proj_d, link_d) = per-query hits on the tuning queries,
+    ref_hits = the FP32 build's), so the accepted beams are exactly the
+    tried ones above the first failure; of those the fastest is taken.
+    ref_link (the FP32 build's link rows) adds the per-node overlap of each
+    beam's link rows with it to the report.  Returns (params, report)."""
+    import numpy as np
     import torch
     if p.precision == _native.PG_FP32:
         raise ValueError("the FP32 path keeps the reference beam mapping")
@@ -273,8 +294,9 @@ def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
     i32 = dict(dtype=torch.int32, device=base_d.device)
     proj = torch.empty((n, p.proj_deg), **i32)
     link = torch.empty((n, p.M), **i32)
-    floor = recall_floor(ref_recall, tol_pt, margin_pt, rel_tol)
-    report = {"ref_recall": ref_recall, "floor": floor, "tried": []}
+    ref_recall = float(np.mean(ref_hits)) / 10.0
+    report = {"ref_recall": round(ref_recall, 4), "tol_pt": tol_pt, "z": z,
+              "tuning_queries": int(np.asarray(ref_hits).size), "tried": []}
     best, best_t = None, float("inf")
     t_start = time.perf_counter()
     for L in Ls:
@@ -284,17 +306,43 @@ def tune_beam(base_d, gt_d, p, recall_fn, ref_recall, Ls=None, tol_pt=0.5,
         with _native.Context(pp) as ctx:
             t = _time(lambda: (ctx.stage_device(base_d, gt_d, proj, link),
                                ctx.wait()), reps=reps)
-        rec = recall_fn(proj, link)
-        ok = rec >= floor
-        report["tried"].append({"L": L, "ms": round(1000 * t, 2),
-                                "recall_at_10": round(rec, 4), "ok": ok})
+        hits = hits_fn(proj, link)
+        ok, delta, lower, tol = beam_accepts(hits, ref_hits, tol_pt, z,
+                                             rel_tol)
+        row = {"L": L, "ms": round(1000 * t, 2),
+               "recall_at_10": round(float(np.mean(hits)) / 10.0, 4),
+               "delta_pt": round(100 * delta, 3),
+               "lower_bound_pt": round(100 * lower, 3), "ok": bool(ok)}
+        if ref_link is not None:
+            row["link_overlap_vs_fp32"] = link_overlap(ref_link, link)
+        report["tried"].append(row)
         if log:
             log(f"autotune beam L={L}: {1000 * t:.1f} ms, recall@10 "
-                f"{rec:.4f} ({'ok' if ok else 'below floor'})")
-        if ok and t < best_t:
+                f"{row['recall_at_10']:.4f} (delta {100 * delta:+.3f} pt, "
+                f"lower bound {100 * lower:+.3f} pt, tolerance "
+                f"-{100 * tol:.2f} pt: {'ok' if ok else 'fails: stop'})")
+        if not ok:
+            break
+        if t < best_t:
             best, best_t = pp, t
     report["seconds"] = round(time.perf_counter() - t_start, 2)
     if best is None:
         best = _native.PgParams.from_buffer_copy(p)
     report["L"] = int(best.Km)
     return best, report
+
+
+def link_overlap(ref_link, link, rows=20000):
+    """Mean per-node overlap |ref & other| / |ref| of two link-row sets
+    (device or host int32 tensors / arrays, tomb = -1), on the first
+    `rows` nodes."""
+    import numpy as np
+    a = ref_link[:rows].cpu().numpy() if hasattr(ref_link, "cpu") else \
+        np.asarray(ref_link[:rows])
+    b = link[:rows].cpu().numpy() if hasattr(link, "cpu") else \
+        np.asarray(link[:rows])
+    tot = 0.0
+    for x, y in zip(a, b):
+        sx = set(x[x >= 0].tolist())
+        tot += len(sx & set(y[y >= 0].tolist())) / max(1, len(sx))
+    return round(tot / max(1, a.shape[0]), 4)

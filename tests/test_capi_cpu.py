@@ -113,12 +113,39 @@ def test_builder_rejects_unknown_precision():
         b.LaunchBuild(M=16, L=16, precision="fp16")
 
 
-def test_beam_recall_floor():
+def test_beam_acceptance_rule():
+    """beam_accepts: paired per-query differences, lower confidence bound
+    against 0.5 pt (or 5% of a small recall)."""
     from paragraph_amd import autotune
-    # absolute 0.3 pt when recall is large, 5% relative when it is small
-    assert abs(autotune.recall_floor(0.12) - 0.117) < 1e-12
-    assert abs(autotune.recall_floor(0.26) - 0.257) < 1e-12
-    assert abs(autotune.recall_floor(0.0066) - 0.0066 * 0.95) < 1e-12
+    rng = np.random.default_rng(3)
+    ref = rng.integers(0, 11, 40000)
+    # identical hits: delta 0, bound 0, accepted
+    ok, d, lo, tol = autotune.beam_accepts(ref, ref)
+    assert ok and d == 0.0 and lo == 0.0 and abs(tol - 0.005) < 1e-12
+    # one hit lost on 2% of the queries: -0.2 pt, tight bound, accepted
+    h = ref.copy()
+    idx = np.flatnonzero(h > 0)[:800]
+    h[idx] -= 1
+    ok, d, lo, _ = autotune.beam_accepts(h, ref)
+    assert ok and abs(d - (-800 / 40000 / 10)) < 1e-12 and -0.005 < lo < d
+    # one hit lost on 6%: -0.6 pt, rejected
+    h = ref.copy()
+    idx = np.flatnonzero(h > 0)[:2400]
+    h[idx] -= 1
+    assert not autotune.beam_accepts(h, ref)[0]
+    # a -0.45 pt mean on few queries is rejected by its wide bound, and
+    # accepted on many queries with the same per-query spread
+    small = np.array([10] * 1000)
+    hs = small.copy()
+    hs[:9] -= 5
+    assert not autotune.beam_accepts(hs, small)[0]
+    big = np.tile(small, 400)
+    hb = np.tile(hs, 400)
+    assert autotune.beam_accepts(hb, big)[0]
+    # small recall: 5% relative cap
+    ref = np.zeros(1000, np.int64)
+    ref[:66] = 1
+    assert abs(autotune.beam_accepts(ref, ref)[3] - 0.05 * 0.0066) < 1e-12
     for L in (128, 96, 64):
         p = _native.make_params(1000, 64, 1000, 32, 128,
                                 precision=_native.PG_BF16)
