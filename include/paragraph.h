@@ -98,7 +98,9 @@ typedef struct pg_params {
                            row, one queue segment per XCD), and the
                            link-row prune its nodes; bit 10 (0x400),
                            with bit 9: the link-row merge takes its nodes
-                           in that order too (same rows) */
+                           in that order too (same rows); bit 11 (0x800):
+                           two waves per search query, the second sharing
+                           its distance evaluations (same rows) */
   uint32_t reserved[7];
 } pg_params;
 
@@ -143,7 +145,13 @@ void pg_destroy(pg_ctx* ctx);
 /* Whole GPU stage from host buffers: H2D of base (N x dim) and gt
  * (n_query x gt_k), the 12-launch pipeline, D2H of the projection
  * (N x proj_deg) and link (N x M) rows (tests/test_t2iend.cu:858-1155).
- * Returns after enqueueing; outputs are valid once pg_wait() returns 0. */
+ * The inputs stream in on the context's copy stream in about 8 chunks each
+ * (256 KiB to 64 MiB per chunk), gt first: the match runs on each gt chunk and the screened
+ * copies of each base chunk are made as it lands; the projection rows
+ * stream out while the link rounds run.  Host buffers from pg_host_alloc
+ * (pinned) copy at full link speed; pageable buffers work too, staged by
+ * the runtime.  Returns after enqueueing; outputs are valid once pg_wait()
+ * returns 0, and the input buffers must stay unchanged until then. */
 int pg_gpu_stage_async(pg_ctx* ctx, const float* base, const uint32_t* gt,
                        uint32_t* out_proj, uint32_t* out_link);
 
@@ -151,6 +159,14 @@ int pg_gpu_stage_async(pg_ctx* ctx, const float* base, const uint32_t* gt,
 int pg_gpu_stage_device_async(pg_ctx* ctx, const float* d_base,
                               const uint32_t* d_gt, uint32_t* d_proj,
                               uint32_t* d_link);
+
+/* Pinned (page-locked) host memory for the inputs and outputs of
+ * pg_gpu_stage_async: copies from and to it run at full link speed and
+ * overlap the kernels.  Replaces the pageable std::vector buffers the
+ * reference copies from (tests/test_t2iend.cu:735-760, 858-873).  Free
+ * with pg_host_free. */
+int pg_host_alloc(uint64_t bytes, void** out);
+int pg_host_free(void* ptr);
 
 int pg_wait(pg_ctx* ctx);   /* blocks until the last enqueued work ends */
 int pg_query(pg_ctx* ctx);  /* PG_OK if done, PG_ERR_BUSY if in flight */
@@ -250,6 +266,24 @@ int pg_run_fusion_final(pg_ctx* ctx, const pg_host_params* hp,
                         uint32_t M_link, uint32_t* d_off, uint32_t* d_ids);
 
 int pg_sync(pg_ctx* ctx);  /* hipStreamSynchronize + error check */
+
+/* The gt the build reads, computed on the GPU: for each query the k <= 256
+ * base ids with the largest inner product, scored by the FP32 sequential
+ * chain (Gbuilder.cuh:785-797's order), ties to the lower id, -1 past
+ * n_base; d_out is n_query x k int32, the layout the reference reads from
+ * its gt file after transform_gt_to_128.ipynb (tests/test_t2iend.cu:
+ * 735-760).  A BF16 MFMA screen with a rigorous per-pair bound keeps a
+ * superset of every top-k, which is re-scored exactly; a query whose
+ * bound admits more than 1024 base rows around its k-th score (crowded
+ * scores) is computed by an exact pass over all base rows instead, and
+ * *n_exact (may be NULL) counts those rows.  dim <= 512.  Runs on `stream`
+ * (a hipStream_t, NULL = the null stream) and returns when done; it
+ * allocates its workspace (BF16 copies and about 8 KiB of candidates per
+ * query, in chunks of 2^20 queries). */
+int pg_exact_topk_ip(const float* d_queries, uint32_t n_query,
+                     const float* d_base, uint32_t n_base, uint32_t dim,
+                     uint32_t k, int32_t* d_out, uint32_t* n_exact,
+                     void* stream);
 
 /* Allocation-free variant of pg_run_search.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code

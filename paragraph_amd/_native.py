@@ -51,6 +51,7 @@ EXPORTED = (
     "pg_run_match", "pg_run_prune", "pg_run_reverse", "pg_run_merge",
     "pg_run_search", "pg_run_pair_distances", "pg_sync", "pg_run_fusion",
     "pg_run_bipartite_async", "pg_run_fusion_final", "pg_search_noalloc",
+    "pg_host_alloc", "pg_host_free", "pg_exact_topk_ip",
 )
 
 _lib = None
@@ -107,6 +108,10 @@ def _declare(L):
                                         ctypes.c_int, _P]
     L.pg_search_noalloc.argtypes = [ctypes.POINTER(PgParams), _P, _U32, _P,
                                     _P, _U32, _P, _P]
+    L.pg_exact_topk_ip.argtypes = [_P, _U32, _P, _U32, _U32, _U32, _P,
+                                   ctypes.POINTER(_U32), _P]
+    L.pg_host_alloc.argtypes = [ctypes.c_uint64, ctypes.POINTER(_P)]
+    L.pg_host_free.argtypes = [_P]
 
 
 def stage_name(i):
@@ -137,6 +142,47 @@ def search_noalloc(params, queries, base, graph, out):
         _dptr(graph), graph.shape[1], _dptr(out), ctypes.c_void_p(stream))
     if rc != 0:
         raise PgError(f"rc={rc}: {lib().pg_last_error(None).decode()}")
+
+
+def exact_topk_ip(queries, base, k, out):
+    """pg_exact_topk_ip: out[q] = the k base ids with the largest inner
+    product (FP32 chain scores, ties to the lower id) for device tensors
+    queries (n x dim f32), base (N x dim f32), out (n x k int32).  Returns
+    the number of rows computed by the exact pass (crowded scores)."""
+    import torch
+    stream = torch.cuda.current_stream(queries.device).cuda_stream
+    n_exact = _U32(0)
+    rc = lib().pg_exact_topk_ip(_dptr(queries), queries.shape[0],
+                                _dptr(base), base.shape[0], base.shape[1], k,
+                                _dptr(out), ctypes.byref(n_exact),
+                                ctypes.c_void_p(stream))
+    if rc != 0:
+        raise PgError(f"rc={rc}: {lib().pg_last_error(None).decode()}")
+    return int(n_exact.value)
+
+
+class PinnedArray:
+    """A numpy array over pinned host memory from pg_host_alloc (freed with
+    pg_host_free when this object goes away); .a is the array."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        ptr = _P()
+        rc = lib().pg_host_alloc(max(n, 1), ctypes.byref(ptr))
+        if rc != 0:
+            raise PgError(f"rc={rc}: {lib().pg_last_error(None).decode()}")
+        self._ptr = ptr.value
+        buf = (ctypes.c_uint8 * max(n, 1)).from_address(self._ptr)
+        self.a = np.frombuffer(buf, dtype=np.uint8, count=n).view(
+            dtype).reshape(shape)
+
+    def __del__(self):
+        if getattr(self, "_ptr", None):
+            self.a = None
+            lib().pg_host_free(_P(self._ptr))
+            self._ptr = None
 
 
 def check_params(p):

@@ -3,6 +3,8 @@
 Knobs (pg_params.tune_flags and table_size):
   - gather batch: FP32 search, candidate pairs in flight (4, 8 or 16);
     screened search, 16-candidate MFMA groups in flight (1 or 2);
+  - waves per search query: 1, or 2 (the second wave shares the distance
+    evaluations);
   - persistent search grid on / off;
   - locality order of the persistent search's queries on / off;
   - split-BF16 screen in the screened prune / merge on / off;
@@ -50,18 +52,23 @@ def _time(fn, reps=2):
     return best
 
 
+WAVES2 = 0x800  # tune_flags bit 11: two waves per search query
+
+
 def candidates(p):
-    """Knob sets to try for params p (tune_flags, table_size)."""
+    """Knob sets to try for params p (tune_flags, table_size): gather
+    batch, waves per query (1 or 2), persistent grid, visited-table size."""
     screened = p.precision != _native.PG_FP32
     kpkd = p.Km + p.Kp * p.Kd
     out = []
     gathers = (1, 2) if screened else (4, 8, 16)
     tables = ([0, 2 * kpkd, 8 * kpkd] if screened else [0])
     for g in gathers:
-        for persistent in (True, False):
-            for ts in tables:
-                tf = g | (0x20 if persistent else 0)
-                out.append({"tune_flags": tf, "table_size": ts})
+        for waves in (0, WAVES2):
+            for persistent in (True, False):
+                for ts in tables:
+                    tf = g | waves | (0x20 if persistent else 0)
+                    out.append({"tune_flags": tf, "table_size": ts})
     return out
 
 
@@ -93,7 +100,7 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     best, best_t = None, float("inf")
     for kn in candidates(p):
         pp = _native.PgParams.from_buffer_copy(p)
-        pp.tune_flags = (p.tune_flags & ~0x3F) | kn["tune_flags"]
+        pp.tune_flags = (p.tune_flags & ~0x83F) | kn["tune_flags"]
         pp.table_size = kn["table_size"]
         if _native.check_params(pp)[0] != 0:
             continue
@@ -113,11 +120,11 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     # sample's gather ranking was seen to flip on the full run)
     if best is not None:
         full_t = {}
-        gathers = sorted({kn["tune_flags"] & 0x1F for kn in candidates(p)})
+        gathers = sorted({kn["tune_flags"] & 0x81F for kn in candidates(p)})
         for g in gathers:
             for grid in (0x20, 0x220):
                 pp = _native.PgParams.from_buffer_copy(p)
-                pp.tune_flags = (p.tune_flags & ~0x23F) | g | grid
+                pp.tune_flags = (p.tune_flags & ~0xA3F) | g | grid
                 pp.table_size = best["table_size"]
                 if _native.check_params(pp)[0] != 0:
                     continue
@@ -136,7 +143,7 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
                         f"rows): {1000 * full_t[g | grid]:.2f} ms")
         if full_t:
             gg = min(full_t, key=full_t.get)
-            best["tune_flags"] = (best["tune_flags"] & ~0x23F) | gg
+            best["tune_flags"] = (best["tune_flags"] & ~0xA3F) | gg
     # prune variants on the search rows of every base row (the prune's cost
     # depends on cache reuse across the whole node range, so a sample with
     # empty rows elsewhere misranks them): kept-major sweep or
@@ -170,13 +177,13 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     # the locality order (bit 9) also orders the link-row prune: decide it
     # on search + prune time with the chosen gather batch and prune variant
     if best is not None and full_t:
-        g = best["tune_flags"] & 0x1F
+        g = best["tune_flags"] & 0x81F
         tot = {}
         for grid in (0x20, 0x220):
             if (g | grid) not in full_t:
                 continue
             pp = _native.PgParams.from_buffer_copy(p)
-            pp.tune_flags = ((p.tune_flags & ~0x3BF) | g | grid | best_pf)
+            pp.tune_flags = ((p.tune_flags & ~0xBBF) | g | grid | best_pf)
             with _native.Context(pp) as ctx:
                 t = _time(lambda: (ctx.run_prune_rows(base_d, full, p.M, pr,
                                                       pc), ctx.sync()))
@@ -195,7 +202,7 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     # the reverse lists of the pruned rows of every base row
     if best["tune_flags"] & 0x200:
         pp = _native.PgParams.from_buffer_copy(p)
-        pp.tune_flags = (p.tune_flags & ~0x7BF) | best["tune_flags"]
+        pp.tune_flags = (p.tune_flags & ~0xFBF) | best["tune_flags"]
         off = torch.empty(n + 1, **i32)
         srcs = torch.empty(n * p.M, **i32)
         out = torch.empty((n, p.M), **i32)
@@ -205,7 +212,7 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             ctx.sync()
         mt = {}
         for mo in (0, 0x400):
-            pp.tune_flags = ((p.tune_flags & ~0x7BF) | best["tune_flags"]
+            pp.tune_flags = ((p.tune_flags & ~0xFBF) | best["tune_flags"]
                              | mo)
             with _native.Context(pp) as ctx:
                 mt[mo] = _time(lambda: (ctx.run_merge(
@@ -234,7 +241,7 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
 def apply(p, best):
     """Copy of p with the tuned knobs."""
     pp = _native.PgParams.from_buffer_copy(p)
-    pp.tune_flags = (p.tune_flags & ~0x7BF) | best["tune_flags"]
+    pp.tune_flags = (p.tune_flags & ~0xFBF) | best["tune_flags"]
     pp.table_size = best["table_size"]
     return pp
 

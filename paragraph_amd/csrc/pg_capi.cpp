@@ -84,6 +84,12 @@ struct pg_ctx {
   uint32_t* h_gt_dev = nullptr;
   uint32_t* h_proj_dev = nullptr;
   uint32_t* h_link_dev = nullptr;
+  // its copy stream and events: inputs stream in chunk by chunk beside the
+  // first kernels, the projection rows stream out beside the link rounds
+  hipStream_t copy = nullptr;
+  hipEvent_t ev_chunk[8] = {};
+  hipEvent_t ev_proj = nullptr;
+  hipEvent_t ev_out = nullptr;
   char err[512] = "";
 };
 
@@ -156,13 +162,19 @@ uint32_t effective_table_size(const pg_params& p) {
 //             (nodes grouped by nearest pivot row, one queue segment per
 //             XCD), and the link-row prune its nodes; same rows
 //   bit 10    with bit 9, the merge takes its nodes in that order too
+//   bit 11    two waves per search query (the second shares the distance
+//             evaluations); same rows
 constexpr uint32_t kTunePersistent = 0x20u;
 constexpr uint32_t kTuneGraph = 0x40u;
 constexpr uint32_t kTuneSplit = 0x80u;
 constexpr uint32_t kTuneScanPrune = 0x100u;
 constexpr uint32_t kTuneOrder = 0x200u;
 constexpr uint32_t kTuneOrderMerge = 0x400u;
+constexpr uint32_t kTuneWaves2 = 0x800u;
 uint32_t tune_gather(const pg_params& p) { return p.tune_flags & 0x1Fu; }
+uint32_t tune_waves(const pg_params& p) {
+  return (p.tune_flags & kTuneWaves2) ? 2u : 1u;
+}
 
 // Bound factor of the BF16 screen against the exact FP32 chain:
 // |screened - chain| <= eps * sum|x_i y_i| <= eps * |x| |y| with
@@ -263,8 +275,13 @@ int search_order(pg_ctx* c, const float* d_rows, uint32_t n,
   return PG_OK;
 }
 
+// The whole stage on the context's stream.  prologue_done: the screened
+// copies of the base and the match were already enqueued (the host entry
+// point runs them chunk by chunk as the inputs arrive); proj_ready, when
+// given, is recorded once the projection rows are final.
 int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
-                 uint32_t* d_proj, uint32_t* d_link) {
+                 uint32_t* d_proj, uint32_t* d_link,
+                 bool prologue_done = false, hipEvent_t proj_ready = nullptr) {
   const pg_params& p = c->p;
   const uint32_t N = p.n_base;
   hipStream_t s = c->stream;
@@ -272,9 +289,11 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   PG_TRY(record(c, ev++));
   const pg::ScreenArgs sc = screen_args(c);
   const pg::ScreenArgs* scp = c->base16 ? &sc : nullptr;
-  if (c->base16)  // screened path: BF16 split of the base and row norms
-    PG_TRY(prepare_screen(c, d_base));
-  PG_HIP(c, pg::launch_match(d_gt, p.n_query, p.gt_k, N, c->winner, s));
+  if (!prologue_done) {
+    if (c->base16)  // screened path: BF16 split of the base and row norms
+      PG_TRY(prepare_screen(c, d_base));
+    PG_HIP(c, pg::launch_match(d_gt, p.n_query, p.gt_k, N, c->winner, s));
+  }
   PG_TRY(record(c, ev++));
   PG_HIP(c, pg::launch_prune(d_base, N, p.dim, nullptr, p.gt_k, d_gt,
                              c->winner, p.gt_k, p.proj_pr, c->pruned, c->pcnt,
@@ -286,6 +305,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   PG_HIP(c, pg::launch_merge(d_base, N, p.dim, c->off, c->srcs, p.proj_rev,
                              c->pruned, c->pcnt, p.proj_pr, p.proj_deg, 0,
                              d_proj, c->stats, nullptr, nullptr, s));
+  if (proj_ready) PG_HIP(c, hipEventRecord(proj_ready, s));
   PG_TRY(record(c, ev++));
   const uint32_t* graph = d_proj;
   uint32_t gdeg = p.proj_deg;
@@ -306,7 +326,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
                                 c->res, c->stats,
                                 (p.tune_flags & kTunePersistent) ? c->queue
                                                                  : nullptr,
-                                sorder, s));
+                                sorder, tune_waves(p), s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, width, nullptr,
                                nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
@@ -561,6 +581,15 @@ void pg_destroy(pg_ctx* c) {
     (void)hipStreamDestroy(c->aux);
     c->aux = nullptr;
   }
+  if (c->copy) {
+    (void)hipStreamSynchronize(c->copy);
+    (void)hipStreamDestroy(c->copy);
+    c->copy = nullptr;
+  }
+  for (hipEvent_t& e : c->ev_chunk)
+    if (e) (void)hipEventDestroy(e);
+  if (c->ev_proj) (void)hipEventDestroy(c->ev_proj);
+  if (c->ev_out) (void)hipEventDestroy(c->ev_out);
   if (c->ev_in) (void)hipEventDestroy(c->ev_in);
   if (c->ev_bip) (void)hipEventDestroy(c->ev_bip);
   c->ev_in = c->ev_bip = nullptr;
@@ -625,23 +654,133 @@ int pg_gpu_stage_async(pg_ctx* c, const float* base, const uint32_t* gt,
     PG_TRY(dev_alloc(c, &c->h_proj_dev, N * p.proj_deg, "proj"));
     PG_TRY(dev_alloc(c, &c->h_link_dev, N * p.M, "link"));
   }
-  PG_HIP(c, hipMemcpyAsync(c->h_base_dev, base, N * p.dim * sizeof(float),
-                           hipMemcpyHostToDevice, c->stream));
-  PG_HIP(c, hipMemcpyAsync(c->h_gt_dev, gt,
-                           uint64_t(p.n_query) * p.gt_k * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, c->stream));
-  if (c->p.tune_flags & kTuneGraph)
+  if (c->p.tune_flags & kTuneGraph) {
+    // the captured pipeline is replayed whole: plain copies around it
+    PG_HIP(c, hipMemcpyAsync(c->h_base_dev, base, N * p.dim * sizeof(float),
+                             hipMemcpyHostToDevice, c->stream));
+    PG_HIP(c, hipMemcpyAsync(c->h_gt_dev, gt,
+                             uint64_t(p.n_query) * p.gt_k * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, c->stream));
     PG_TRY(run_pipeline_graph(c, c->h_base_dev, c->h_gt_dev, c->h_proj_dev,
                               c->h_link_dev));
-  else
-    PG_TRY(run_pipeline(c, c->h_base_dev, c->h_gt_dev, c->h_proj_dev,
-                        c->h_link_dev));
+    PG_HIP(c, hipMemcpyAsync(out_proj, c->h_proj_dev,
+                             N * p.proj_deg * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    PG_HIP(c, hipMemcpyAsync(out_link, c->h_link_dev,
+                             N * p.M * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    PG_HIP(c, hipEventRecord(c->done, c->stream));
+    return PG_OK;
+  }
+  if (!c->copy) {
+    PG_HIP(c, hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking));
+    for (hipEvent_t& e : c->ev_chunk)
+      PG_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    PG_HIP(c, hipEventCreateWithFlags(&c->ev_proj, hipEventDisableTiming));
+    PG_HIP(c, hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+  }
+  // Inputs stream in on the copy stream in about 8 chunks each (256 KiB to
+  // 64 MiB per chunk): the gt first (the match runs on each chunk as it
+  // lands), then the base (the screened copies are made per chunk).  Copies from pinned host memory
+  // (pg_host_alloc) run at full link speed; pageable memory also works,
+  // staged by the runtime.  The copy stream first waits for the main
+  // stream, so a previous build's kernels are done with the buffers.
+  hipStream_t s = c->stream;
+  PG_HIP(c, hipEventRecord(c->ev_out, s));
+  PG_HIP(c, hipStreamWaitEvent(c->copy, c->ev_out, 0));
+  PG_HIP(c, pg::launch_fill_u32(c->winner, N, PG_TOMB, s));
+  uint32_t next_ev = 0;
+  auto chunked = [&](uint64_t rows, uint64_t row_bytes, auto&& copy_rows,
+                     auto&& on_rows) -> int {
+    uint64_t chunk = rows * row_bytes / 8;
+    chunk = chunk < (256ull << 10) ? (256ull << 10)
+            : chunk > (64ull << 20) ? (64ull << 20) : chunk;
+    const uint64_t per = row_bytes >= chunk ? 1 : chunk / row_bytes;
+    for (uint64_t r0 = 0; r0 < rows; r0 += per) {
+      const uint64_t nr = rows - r0 < per ? rows - r0 : per;
+      PG_TRY(copy_rows(r0, nr));
+      hipEvent_t e = c->ev_chunk[next_ev++ % 8];
+      PG_HIP(c, hipEventRecord(e, c->copy));
+      PG_HIP(c, hipStreamWaitEvent(s, e, 0));
+      PG_TRY(on_rows(r0, nr));
+    }
+    return PG_OK;
+  };
+  const uint64_t gt_row = uint64_t(p.gt_k) * sizeof(uint32_t);
+  PG_TRY(chunked(
+      p.n_query, gt_row,
+      [&](uint64_t r0, uint64_t nr) -> int {
+        PG_HIP(c, hipMemcpyAsync(c->h_gt_dev + r0 * p.gt_k, gt + r0 * p.gt_k,
+                                 nr * gt_row, hipMemcpyHostToDevice, c->copy));
+        return PG_OK;
+      },
+      [&](uint64_t r0, uint64_t nr) -> int {
+        PG_HIP(c, pg::launch_match_rows(c->h_gt_dev + r0 * p.gt_k,
+                                        uint32_t(r0), uint32_t(nr), p.gt_k,
+                                        p.n_base, c->winner, s));
+        return PG_OK;
+      }));
+  const uint64_t base_row = uint64_t(p.dim) * sizeof(float);
+  PG_TRY(chunked(
+      N, base_row,
+      [&](uint64_t r0, uint64_t nr) -> int {
+        PG_HIP(c, hipMemcpyAsync(c->h_base_dev + r0 * p.dim, base + r0 * p.dim,
+                                 nr * base_row, hipMemcpyHostToDevice,
+                                 c->copy));
+        return PG_OK;
+      },
+      [&](uint64_t r0, uint64_t nr) -> int {
+        if (!c->base16) return PG_OK;
+        const uint64_t d = c->dpad;
+        PG_HIP(c, pg::launch_prepare_screen(
+                      c->h_base_dev + r0 * p.dim, uint32_t(nr), p.dim, c->dpad,
+                      static_cast<uint16_t*>(c->base16) + r0 * d,
+                      c->base16lo ? static_cast<uint16_t*>(c->base16lo) + r0 * d
+                                  : nullptr,
+                      c->norms + r0,
+                      c->base16lo ? nullptr : c->rnorms + r0, s));
+        if (c->base8)
+          PG_HIP(c, pg::launch_prepare_fp8(
+                        c->h_base_dev + r0 * p.dim, uint32_t(nr), p.dim,
+                        c->dpad, static_cast<uint8_t*>(c->base8) + r0 * d,
+                        c->scale8 + r0, s));
+        return PG_OK;
+      }));
+  PG_TRY(run_pipeline(c, c->h_base_dev, c->h_gt_dev, c->h_proj_dev,
+                      c->h_link_dev, /*prologue_done=*/true, c->ev_proj));
+  // the projection rows leave while the link rounds run
+  PG_HIP(c, hipStreamWaitEvent(c->copy, c->ev_proj, 0));
   PG_HIP(c, hipMemcpyAsync(out_proj, c->h_proj_dev,
                            N * p.proj_deg * sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, c->stream));
+                           hipMemcpyDeviceToHost, c->copy));
   PG_HIP(c, hipMemcpyAsync(out_link, c->h_link_dev, N * p.M * sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, c->stream));
-  PG_HIP(c, hipEventRecord(c->done, c->stream));
+                           hipMemcpyDeviceToHost, s));
+  // the main stream's completion covers the copy stream too
+  PG_HIP(c, hipEventRecord(c->ev_out, c->copy));
+  PG_HIP(c, hipStreamWaitEvent(s, c->ev_out, 0));
+  PG_HIP(c, hipEventRecord(c->done, s));
+  return PG_OK;
+}
+
+int pg_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return set_err(nullptr, PG_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (bytes == 0) return set_err(nullptr, PG_ERR_INVALID, "zero bytes");
+  const hipError_t e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return set_err(nullptr, PG_ERR_OOM, "hipHostMalloc(%llu bytes): %s",
+                   (unsigned long long)bytes, hipGetErrorString(e));
+  }
+  return PG_OK;
+}
+
+int pg_host_free(void* ptr) {
+  if (!ptr) return PG_OK;
+  const hipError_t e = hipHostFree(ptr);
+  if (e != hipSuccess)
+    return set_err(nullptr, PG_ERR_HIP, "hipHostFree: %s",
+                   hipGetErrorString(e));
   return PG_OK;
 }
 
@@ -808,7 +947,7 @@ int pg_run_search(pg_ctx* c, const float* d_queries, uint32_t n_query,
                               c->stats,
                               (p.tune_flags & kTunePersistent) ? c->queue
                                                                : nullptr,
-                              sorder, c->stream));
+                              sorder, tune_waves(p), c->stream));
   return PG_OK;
 }
 
@@ -1062,7 +1201,7 @@ int pg_search_noalloc(const pg_params* p, const float* d_queries,
       p->dim, d_graph, graph_deg, p->Km, p->Kp, p->Kd, p->topk, p->reset_iter,
       p->entry_seed, p->max_iters, effective_table_size(*p), tune_gather(*p),
       d_out, nullptr, persistent ? queue : nullptr, ordered ? order : nullptr,
-      s);
+      tune_waves(*p), s);
   if (e != hipSuccess)
     return set_err(nullptr, PG_ERR_HIP, "search launch: %s",
                    hipGetErrorString(e));
@@ -1070,3 +1209,93 @@ int pg_search_noalloc(const pg_params* p, const float* d_queries,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Exact top-k gt (pg_exact_topk_ip).
+extern "C" int pg_exact_topk_ip(const float* d_queries, uint32_t n_query,
+                                const float* d_base, uint32_t n_base,
+                                uint32_t dim, uint32_t k, int32_t* d_out,
+                                uint32_t* n_exact, void* stream) {
+  if (n_exact) *n_exact = 0;
+  if ((n_query && (!d_queries || !d_out)) || !d_base)
+    return set_err(nullptr, PG_ERR_INVALID, "NULL argument");
+  if (dim == 0 || dim > 512)
+    return set_err(nullptr, PG_ERR_INVALID, "dim must be in [1, 512]");
+  if (k == 0 || k > 256)
+    return set_err(nullptr, PG_ERR_INVALID, "k must be in [1, 256]");
+  if (n_base == 0 || n_base >= 0x80000000u)
+    return set_err(nullptr, PG_ERR_INVALID, "n_base must be in [1, 2^31)");
+  if (n_query == 0) return PG_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t dpad = (dim + 31) / 32 * 32;
+  const uint32_t chunk = n_query < (1u << 20) ? n_query : (1u << 20);
+  // accumulation (dpad terms, 2^-23) and chain (dim terms, 2^-24) factors
+  const double ua = std::ldexp(1.0, -23), uc = std::ldexp(1.0, -24);
+  const float gam_a = float(dpad * ua / (1.0 - dpad * ua));
+  const float gam_c = float(dim * uc / (1.0 - dim * uc));
+  uint16_t* q16 = nullptr;
+  uint16_t* b16 = nullptr;
+  float* qn = nullptr;
+  float* bn = nullptr;
+  float* qr = nullptr;
+  float* br = nullptr;
+  uint2* buf = nullptr;
+  uint32_t* cnt = nullptr;
+  uint32_t* list = nullptr;
+  uint32_t* nlist = nullptr;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(s);
+    for (void* p : {static_cast<void*>(q16), static_cast<void*>(b16),
+                    static_cast<void*>(qn), static_cast<void*>(bn),
+                    static_cast<void*>(qr), static_cast<void*>(br),
+                    static_cast<void*>(buf), static_cast<void*>(cnt),
+                    static_cast<void*>(list), static_cast<void*>(nlist)})
+      if (p) (void)hipFree(p);
+  };
+  auto alloc = [&](auto** ptr, uint64_t bytes, const char* what) -> int {
+    if (hipMalloc(reinterpret_cast<void**>(ptr), bytes) != hipSuccess) {
+      *ptr = nullptr;
+      return set_err(nullptr, PG_ERR_OOM, "hipMalloc(%s, %llu bytes)", what,
+                     (unsigned long long)bytes);
+    }
+    return PG_OK;
+  };
+  int rc = PG_OK;
+  if ((rc = alloc(&q16, uint64_t(chunk) * dpad * 2, "queries_bf16")) ||
+      (rc = alloc(&b16, uint64_t(n_base) * dpad * 2, "base_bf16")) ||
+      (rc = alloc(&qn, uint64_t(chunk) * 4, "query_norms")) ||
+      (rc = alloc(&bn, uint64_t(n_base) * 4, "base_norms")) ||
+      (rc = alloc(&qr, uint64_t(chunk) * 4, "query_residual_norms")) ||
+      (rc = alloc(&br, uint64_t(n_base) * 4, "base_residual_norms")) ||
+      (rc = alloc(&buf, uint64_t(chunk) * pg::kTopkBufferPairs * 8,
+                  "candidates")) ||
+      (rc = alloc(&cnt, uint64_t(chunk) * 4, "counts")) ||
+      (rc = alloc(&list, uint64_t(chunk) * 4, "exact_rows")) ||
+      (rc = alloc(&nlist, 4, "exact_row_count"))) {
+    release();
+    return rc;
+  }
+  hipError_t e = pg::launch_prepare_screen(d_base, n_base, dim, dpad, b16,
+                                           nullptr, bn, br, s);
+  uint64_t exact_rows = 0;
+  for (uint32_t q0 = 0; e == hipSuccess && q0 < n_query; q0 += chunk) {
+    const uint32_t nq = n_query - q0 < chunk ? n_query - q0 : chunk;
+    e = pg::launch_prepare_screen(d_queries + uint64_t(q0) * dim, nq, dim,
+                                  dpad, q16, nullptr, qn, qr, s);
+    if (e == hipSuccess)
+      e = pg::launch_exact_topk(d_queries + uint64_t(q0) * dim, nq, d_base,
+                                n_base, dim, k, q16, qn, qr, b16, bn, br,
+                                dpad, gam_a, gam_c, buf, cnt, list, nlist,
+                                d_out + uint64_t(q0) * k, s);
+    uint32_t nl = 0;
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&nl, nlist, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    exact_rows += nl;
+  }
+  release();
+  if (e != hipSuccess)
+    return set_err(nullptr, PG_ERR_HIP, "exact top-k: %s", hipGetErrorString(e));
+  if (n_exact) *n_exact = uint32_t(exact_rows);
+  return PG_OK;
+}

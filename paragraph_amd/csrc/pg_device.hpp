@@ -97,10 +97,68 @@ __device__ __forceinline__ float chain_dist(const float* __restrict__ x,
   return -s;
 }
 
+// Value held by lane (lane ^ M), M in {1, 2, 4, 8, 16, 32}, without an LDS
+// round trip: DPP quad permutations for 1 and 2, DPP row rotate by 8 for 8
+// (within a 16-lane row, rotating by 8 is xor 8), ds_swizzle in bit-mask
+// mode for 4 (xor inside 32-lane halves; no address register), and the
+// gfx950 lane swaps v_permlane16_swap / v_permlane32_swap for 16 and 32
+// (each swap hands one of its two results the partner row's values).
+template <uint32_t M>
+__device__ __forceinline__ uint32_t xor_lane32(uint32_t v) {
+  static_assert(M == 1 || M == 2 || M == 4 || M == 8 || M == 16 || M == 32,
+                "lane xor mask");
+  if constexpr (M == 1) {
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0xB1, 0xF, 0xF, false));
+  } else if constexpr (M == 2) {
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x4E, 0xF, 0xF, false));
+  } else if constexpr (M == 4) {
+    return uint32_t(__builtin_amdgcn_ds_swizzle(int(v), 0x101F));
+  } else if constexpr (M == 8) {
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(v), 0x128, 0xF, 0xF, false));
+  } else if constexpr (M == 16) {
+    // rows 0 and 2 read the odd rows from the second result, rows 1 and 3
+    // the even rows from the first
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane_id() & 16u) ? p[0] : p[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane_id() & 32u) ? p[0] : p[1];
+  }
+}
+// Any 4- or 8-byte value, by its bits.
+template <uint32_t M, typename T>
+__device__ __forceinline__ T xor_lane(T v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, xor_lane32<M>(__builtin_bit_cast(uint32_t, v)));
+  } else {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint64_t r = (uint64_t(xor_lane32<M>(uint32_t(u >> 32))) << 32) |
+                       xor_lane32<M>(uint32_t(u));
+    return __builtin_bit_cast(T, r);
+  }
+}
+// Same with the mask as a value that is a constant after unrolling.
+template <typename T>
+__device__ __forceinline__ T xor_lane_dyn(T v, uint32_t m) {
+  switch (m) {
+    case 1: return xor_lane<1, T>(v);
+    case 2: return xor_lane<2, T>(v);
+    case 4: return xor_lane<4, T>(v);
+    case 8: return xor_lane<8, T>(v);
+    case 16: return xor_lane<16, T>(v);
+    default: return xor_lane<32, T>(v);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v += xor_lane<32, T>(v);
+  v += xor_lane<16, T>(v);
+  v += xor_lane<8, T>(v);
+  v += xor_lane<4, T>(v);
+  v += xor_lane<2, T>(v);
+  v += xor_lane<1, T>(v);
   return v;
 }
 
@@ -108,10 +166,30 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (1ull << lane_id()) - 1ull;
 }
 
+// Orders the LDS traffic of one wave between its lanes without a
+// workgroup barrier: a wave's LDS instructions execute in issue order, so
+// it is enough that the compiler keeps its loads and stores on their side
+// of this point.  Unlike __syncthreads it waits for nothing (global loads
+// stay in flight) and leaves the other waves of the workgroup alone.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Barrier of the kernels whose workgroup is one wave (__syncthreads), or,
+// with kWaveSync, of one wave inside a larger workgroup (wave_sync).
+template <bool kWaveSync>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (kWaveSync) wave_sync();
+  else __syncthreads();
+}
+
 // Ascending bitonic sort of n (power of two, >= 2) keys in LDS by one wave
-// (the workgroup is exactly one wave, so __syncthreads orders LDS traffic).
-template <typename K>
+// (lds_sync orders the LDS traffic between its stages).
+template <typename K, bool kWaveSync = false>
 __device__ void bitonic_sort(K* a, uint32_t n) {
+  static_assert(kWave == 64, "lanes stride by the wave size");
   const uint32_t lane = lane_id();
   const uint32_t half = n >> 1;
   for (uint32_t k = 2; k <= n; k <<= 1) {
@@ -126,27 +204,26 @@ __device__ void bitonic_sort(K* a, uint32_t n) {
           a[p] = x;
         }
       }
-      __syncthreads();
+      lds_sync<kWaveSync>();
     }
   }
 }
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, uint32_t m) {
-  const uint32_t lo = __shfl_xor(uint32_t(v), int(m), 64);
-  const uint32_t hi = __shfl_xor(uint32_t(v >> 32), int(m), 64);
-  return (uint64_t(hi) << 32) | lo;
+  return xor_lane_dyn(v, m);
 }
 
 __device__ __forceinline__ uint32_t shfl_xor_key(uint32_t v, uint32_t m) {
-  return __shfl_xor(v, int(m), 64);
+  return xor_lane_dyn(v, m);
 }
 __device__ __forceinline__ uint64_t shfl_xor_key(uint64_t v, uint32_t m) {
-  return shfl_xor_u64(v, m);
+  return xor_lane_dyn(v, m);
 }
 
-// Ascending bitonic sort of 64*R keys (uint32_t or uint64_t) held in
-// registers, element index lane*R + r.  Cross-lane steps exchange through
-// __shfl_xor (no LDS).
+// Ascending bitonic sort (Batcher) of 64*R keys (uint32_t or uint64_t) held
+// in registers, element index lane*R + r.  Cross-lane compare-exchanges take
+// the partner through xor_lane (DPP / ds_swizzle / lane swaps: no LDS
+// traffic and no address registers); in-lane ones swap registers.
 template <int R, typename K>
 __device__ __forceinline__ void reg_bitonic_sort(K (&v)[R]) {
   const uint32_t lane = lane_id();

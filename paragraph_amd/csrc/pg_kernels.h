@@ -32,6 +32,11 @@ hipError_t launch_snapshot_stats(const unsigned long long* stats,
                                  unsigned long long* row, hipStream_t s);
 hipError_t launch_match(const uint32_t* gt, uint32_t n_query, uint32_t gt_k,
                         uint32_t n_base, uint32_t* winner, hipStream_t s);
+// The match over gt rows of queries q0 .. q0 + n_query - 1 (gt points at
+// the first of them) into a winner array already filled with tomb.
+hipError_t launch_match_rows(const uint32_t* gt, uint32_t q0, uint32_t n_query,
+                             uint32_t gt_k, uint32_t n_base, uint32_t* winner,
+                             hipStream_t s);
 hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
                         const uint32_t* rows, uint32_t D, const uint32_t* gt,
                         const uint32_t* winner, uint32_t gt_k, uint32_t mp,
@@ -73,6 +78,8 @@ hipError_t launch_locality_order(const float* base, uint32_t n, uint32_t dim,
 // base8 != nullptr: FP8-screened search over per-row-scaled e4m3 rows.
 // order != nullptr (persistent grid only): queries are taken in that order,
 // kQueueSegs segments with a counter each in queue[0 .. kQueueSegs).
+// waves: waves per query (1, or 2: a second wave shares the distance
+// evaluations of each query; same results).
 hipError_t launch_search(const float* queries, uint32_t n_query,
                          const float* base, const void* base16,
                          const void* base8, const float* scale8, uint32_t dpad,
@@ -83,7 +90,8 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t max_iters, uint32_t table_size,
                          uint32_t gather_batch, uint32_t* out,
                          unsigned long long* stats, uint32_t* queue,
-                         const uint32_t* order, hipStream_t s);
+                         const uint32_t* order, uint32_t waves,
+                         hipStream_t s);
 
 // Host stages of the full build on the GPU (MatchSup rounds 1..rounds,
 // FusionNN, FusionFinal; the lists of libparagraph_host).  Needs
@@ -107,6 +115,25 @@ hipError_t launch_fusion_final(const FusionShape& f, const float* base,
                                const uint32_t* proj, const uint32_t* link,
                                uint32_t* ws, uint32_t* off_out,
                                uint32_t* ids_out, hipStream_t s);
+
+// Exact top-k by inner product (pg_topk.hip): BF16 MFMA screen with the
+// per-pair bound of the screened prune over the BF16 copies (q16 / b16,
+// rows of dpad elements; FP32 norms and residual norms |x - bf16(x)| per
+// row; gam_a / gam_c the accumulation and chain factors), then the
+// survivors re-scored with the FP32 chain; out: nq x k ids (score desc, id
+// asc), -1 past the survivors.  A query whose bound admits more pairs than
+// its buffer holds gets an exact pass over all base rows instead (list: nq
+// words, n_list: one word, the number of such rows after the call).
+// buf: nq x kTopkBufferPairs pairs, count: nq words.
+hipError_t launch_exact_topk(const float* queries, uint32_t nq,
+                             const float* base, uint32_t nb, uint32_t dim,
+                             uint32_t k, const void* q16, const float* qnorm,
+                             const float* qres, const void* b16,
+                             const float* bnorm, const float* bres,
+                             uint32_t dpad, float gam_a, float gam_c,
+                             uint2* buf, uint32_t* count, uint32_t* list,
+                             uint32_t* n_list, int32_t* out, hipStream_t s);
+constexpr uint32_t kTopkBufferPairs = 1024;
 
 // Pair distances: chain != 0 gives the FP32 prune chain; else the search
 // evaluator selected like launch_search (k <= 1024).

@@ -54,14 +54,16 @@ __global__ __launch_bounds__(256) void snapshot_stats_kernel(
 // atomicCAS pick the owner of a contested base row; here the lowest query id
 // wins (rule C1) via atomicMin on a per-base winner array.  The row itself
 // is never materialised: the prune step reads gt[winner][1..gt_k-2].
+// gt holds the rows of queries q0 .. q0 + n_query - 1 (a chunk of the gt
+// when the host streams it in).
 __global__ void match_winner_kernel(const uint32_t* __restrict__ gt,
-                                    uint32_t n_query, uint32_t gt_k,
-                                    uint32_t n_base,
+                                    uint32_t q0, uint32_t n_query,
+                                    uint32_t gt_k, uint32_t n_base,
                                     uint32_t* __restrict__ winner) {
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n_query;
        q += gridDim.x * blockDim.x) {
     uint32_t b = gt[size_t(q) * gt_k];
-    if (b < n_base) atomicMin(&winner[b], q);
+    if (b < n_base) atomicMin(&winner[b], q0 + q);
   }
 }
 
@@ -136,7 +138,7 @@ __device__ uint32_t rng_select(const float* __restrict__ base, uint32_t n_base,
     }
     pairs += wave_sum<unsigned long long>(
         (active && int(lane) <= cut) ? my_pairs : 0u);
-    __syncthreads();
+    wave_sync();
   }
   *pair_total += pairs;
   return nk;
@@ -146,26 +148,26 @@ __device__ uint32_t rng_select(const float* __restrict__ base, uint32_t n_base,
 // registers (64*R keys, shuffle exchanges, no barriers between stages) for n
 // up to 512, else the LDS bitonic sort.  The caller's last LDS writes must be
 // behind a barrier.
-template <int R, typename K>
+template <int R, bool kWaveSync = false, typename K>
 __device__ __forceinline__ void reg_sort_lds(K* a) {
   const uint32_t lane = lane_id();
   K v[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) v[r] = a[lane * R + r];
   reg_bitonic_sort<R>(v);
-  __syncthreads();
+  lds_sync<kWaveSync>();
 #pragma unroll
   for (int r = 0; r < R; ++r) a[lane * R + r] = v[r];
-  __syncthreads();
+  lds_sync<kWaveSync>();
 }
 
-template <typename K>
+template <bool kWaveSync = false, typename K>
 __device__ __forceinline__ void sort_list(K* a, uint32_t n) {
-  if (n == 64) reg_sort_lds<1>(a);
-  else if (n == 128) reg_sort_lds<2>(a);
-  else if (n == 256) reg_sort_lds<4>(a);
-  else if (n == 512) reg_sort_lds<8>(a);
-  else bitonic_sort(a, n);
+  if (n == 64) reg_sort_lds<1, kWaveSync>(a);
+  else if (n == 128) reg_sort_lds<2, kWaveSync>(a);
+  else if (n == 256) reg_sort_lds<4, kWaveSync>(a);
+  else if (n == 512) reg_sort_lds<8, kWaveSync>(a);
+  else bitonic_sort<K, kWaveSync>(a, n);
 }
 
 // Screened RNG (the BF16 path): the same decisions as rng_select, with most
@@ -342,7 +344,7 @@ __device__ uint32_t rng_select_screened(
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       gs[(4u * kq + j) * GW + NKT * 16u + m] = acc_s[j];
-    __syncthreads();
+    wave_sync();
     // ---- resolution: lane i < 16 owns block candidate i (the first
     // occluder in kept order ends a candidate's scan early)
     const bool own = lane < 16 && active;
@@ -398,7 +400,7 @@ __device__ uint32_t rng_select_screened(
     }
     pairs += wave_sum<unsigned long long>((own && int(lane) <= cut) ? my_pairs : 0u);
     fb += wave_sum<unsigned long long>(own ? my_fb : 0u);
-    __syncthreads();
+    wave_sync();
   }
   *pair_total += pairs;
   *fb_total += fb;
@@ -457,14 +459,14 @@ __global__ __launch_bounds__(64) void prune_kernel(
     if (valid) vpos[nv + __popcll(vm & lanemask_lt())] = uint16_t(j);
     nv += __popcll(vm);
   }
-  __syncthreads();
+  wave_sync();
   for (uint32_t v = lane; v < nv; v += kWave) {
     const uint32_t j = vpos[v];
     const float d = chain_dist(xb, base + size_t(sid[j]) * dim, dim);
     skey[j] = (uint64_t(ordered_bits(d)) << 32) | j;
   }
-  __syncthreads();
-  sort_list(skey, DP2);
+  wave_sync();
+  sort_list<true>(skey, DP2);
   unsigned long long pairs = 0, fbk = 0;
   uint32_t nk = 0;
   const uint32_t id0 = sid[uint32_t(skey[0])];
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(64) void prune_kernel(
         kept_nr[0] = make_float2(sc.norms[id0],
                                  sc.rnorms ? sc.rnorms[id0] : 0.0f);
     }
-    __syncthreads();
+    wave_sync();
     if constexpr (NCH == 0)
       nk = rng_select<true>(base, n_base, dim, skey, sid, D, mp, kept, 1,
                             &pairs);
@@ -603,7 +605,7 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
 
 __device__ __forceinline__ float wave_min_f32(float v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, xor_lane_dyn(v, uint32_t(o)));
   return v;
 }
 
@@ -767,7 +769,7 @@ __device__ __forceinline__ uint32_t sweep_select(
         L.sv[e] = s;
       }
     }
-    __syncthreads();
+    wave_sync();
   };
 
   // The alive entry with the smallest exact key.  Only entries whose lower
@@ -805,19 +807,19 @@ __device__ __forceinline__ uint32_t sweep_select(
       best = k < best ? k : best;
     }
     best = wave_min_u64(best);
-    __syncthreads();
+    wave_sync();
     return uint32_t(best & 511u);
   };
 
   uint32_t nk = 0, clast = 0;
   load_frag(0, L.eid[c]);
-  __syncthreads();
+  wave_sync();
   sweep(c, true);
   const uint32_t c1 = pick();
   if (c1 != c) {
     c = c1;
     load_frag(0, L.eid[c]);
-    __syncthreads();
+    wave_sync();
     sweep(c, false);
   }
   while (true) {
@@ -828,7 +830,7 @@ __device__ __forceinline__ uint32_t sweep_select(
         L.ef[c] |= 1u;
         ++my_fb;
       }
-      __syncthreads();
+      wave_sync();
     }
     if ((L.ef[c] & 1u) && !(key_of(c) < kstop)) break;
     if (lane == 0) {
@@ -838,7 +840,7 @@ __device__ __forceinline__ uint32_t sweep_select(
     }
     ++nk;
     clast = c;
-    __syncthreads();
+    wave_sync();
     if (nk == cap) break;
     // every alive entry against the new kept entry c
     const float* xc = base + size_t(L.eid[c]) * dim;
@@ -880,7 +882,7 @@ __device__ __forceinline__ uint32_t sweep_select(
         L.rk[e] = uint16_t(nk - 1);
       }
     }
-    __syncthreads();
+    wave_sync();
     uint32_t n2 = 0;
     for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
       const uint32_t i = i0 + lane;
@@ -891,11 +893,11 @@ __device__ __forceinline__ uint32_t sweep_select(
       n2 += __popcll(km);
     }
     nal = n2;
-    __syncthreads();
+    wave_sync();
     if (nal == 0) break;
     c = pick();
     load_frag(0, L.eid[c]);
-    __syncthreads();
+    wave_sync();
     sweep(c, false);
   }
   if constexpr (!kScreen) {
@@ -939,7 +941,7 @@ __global__ __launch_bounds__(64) void prune_sweep_kernel(
     const uint64_t sm = __ballot(j < D && is_stop_id(id, n_base));
     if (sm && jstop == D) jstop = j0 + __ffsll((long long)sm) - 1;
   }
-  __syncthreads();
+  wave_sync();
   uint32_t ne = 0, nv = 0;
   for (uint32_t j0 = 0; j0 < D; j0 += kWave) {
     const uint32_t j = j0 + lane;
@@ -963,7 +965,7 @@ __global__ __launch_bounds__(64) void prune_sweep_kernel(
     }
     ne += __popcll(im);
   }
-  __syncthreads();
+  wave_sync();
   const uint64_t kstop =
       jstop < D ? ((uint64_t(ordered_bits(FLT_MAX)) << 32) | (jstop << 9))
                 : kPadKey;
@@ -1230,20 +1232,20 @@ __global__ __launch_bounds__(64) void merge_kernel(
     uint32_t n2 = 64;
     while (n2 < seg) n2 <<= 1;
     for (uint32_t j = lane; j < n2; j += kWave) buf[j] = j < seg ? srcs[s0 + j] : kTomb;
-    __syncthreads();
-    if (seg > 1) sort_list(buf, n2);
+    wave_sync();
+    if (seg > 1) sort_list<true>(buf, n2);
   } else {
     for (uint32_t j = lane; j < kSegChunk; j += kWave) buf[j] = srcs[s0 + j];
-    __syncthreads();
-    bitonic_sort(buf, kSegChunk);
+    wave_sync();
+    bitonic_sort<uint32_t, true>(buf, kSegChunk);
     for (uint32_t done = kSegChunk; done < seg; done += kSegChunk) {
       for (uint32_t j = lane; j < kSegChunk; j += kWave) {
         if (j >= cap) buf[j] = kTomb;
         const uint32_t g = done + j;
         buf[kSegChunk + j] = g < seg ? srcs[s0 + g] : kTomb;
       }
-      __syncthreads();
-      bitonic_sort(buf, 2 * kSegChunk);
+      wave_sync();
+      bitonic_sort<uint32_t, true>(buf, 2 * kSegChunk);
     }
   }
   // ---- candidate list: reverse part then pruned part
@@ -1259,13 +1261,13 @@ __global__ __launch_bounds__(64) void merge_kernel(
     if (j < n) d = chain_dist(xc, base + size_t(id) * dim, dim);
     skey[j] = j < n ? ((uint64_t(ordered_bits(d)) << 32) | j) : kPadKey;
   }
-  __syncthreads();
+  wave_sync();
   uint32_t* o = out + size_t(c) * deg;
   if (n == 0) {
     for (uint32_t j = lane; j < deg; j += kWave) o[j] = kTomb;
     return;
   }
-  sort_list(skey, 128u);
+  sort_list<true>(skey, 128u);
   unsigned long long pairs = 0, fbk = 0;
   if (lane == 0) {
     kept[0] = sid[uint32_t(skey[0])];
@@ -1273,7 +1275,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
       kept_nr[0] = make_float2(sc.norms[kept[0]],
                                sc.rnorms ? sc.rnorms[kept[0]] : 0.0f);
   }
-  __syncthreads();
+  wave_sync();
   uint32_t nk;
   if constexpr (NCH == 0)
     nk = rng_select<false>(base, n_base, dim, skey, sid, n, deg, kept, 1,
@@ -1286,7 +1288,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
     // mark the first list slot equal to each kept id, then append the
     // unmarked entries in sorted order (duplicates can be appended again).
     for (uint32_t j = lane; j < 128; j += kWave) added[j] = 0;
-    __syncthreads();
+    wave_sync();
     for (uint32_t a = 0; a < nk; ++a) {
       const uint32_t kid = kept[a];
       uint32_t first = 0xFFFFFFFFu;
@@ -1296,7 +1298,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
         if (m) first = j0 + __ffsll((long long)m) - 1;
       }
       if (lane == 0 && first != 0xFFFFFFFFu) added[first] = 1;
-      __syncthreads();
+      wave_sync();
     }
     if (lane == 0) {
       for (uint32_t i = 0; i < n && nk < deg; ++i) {
@@ -1306,7 +1308,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
       }
     }
     nk = __shfl(nk, 0, 64);
-    __syncthreads();
+    wave_sync();
   }
   for (uint32_t j = lane; j < deg; j += kWave) o[j] = j < nk ? kept[j] : kTomb;
   if (lane == 0) {
@@ -1365,6 +1367,11 @@ struct EvalF32 {
 
   __device__ __forceinline__ void load_query(const float* __restrict__ queries,
                                              uint32_t q) {
+    load_query_regs(queries, q);
+  }
+  // the part of the query every wave that evaluates needs (registers)
+  __device__ __forceinline__ void load_query_regs(
+      const float* __restrict__ queries, uint32_t q) {
     const uint32_t l = lane_id() & 31u;
 #pragma unroll
     for (int t = 0; t < TQ; ++t) {
@@ -1373,12 +1380,15 @@ struct EvalF32 {
     }
   }
 
+  // Candidates todo[0, ntodo) split over `parts` waves: wave `part` takes
+  // the candidate pairs p0 = 2U * (part + parts * i).
   __device__ __forceinline__ void eval(const uint32_t* cand,
                                        const uint32_t* todo, uint32_t ntodo,
-                                       uint64_t* tail) const {
+                                       uint64_t* tail, uint32_t part = 0,
+                                       uint32_t parts = 1) const {
     const uint32_t lane = lane_id();
     const uint32_t half = lane >> 5, l = lane & 31u;
-    for (uint32_t p0 = 0; p0 < ntodo; p0 += 2 * U) {
+    for (uint32_t p0 = 2 * U * part; p0 < ntodo; p0 += 2 * U * parts) {
       uint32_t tt[U], cid[U];
       bool ok[U];
       float xv[U][TQ];
@@ -1405,8 +1415,11 @@ struct EvalF32 {
         for (int t = 0; t < TQ; ++t) {
           if (l + 32u * t < dim) s = __builtin_fmaf(qv[t], xv[u][t], s);
         }
-#pragma unroll
-        for (int o = 16; o > 0; o >>= 1) s = s + __shfl_xor(s, o, 64);
+        s = s + xor_lane<16>(s);
+        s = s + xor_lane<8>(s);
+        s = s + xor_lane<4>(s);
+        s = s + xor_lane<2>(s);
+        s = s + xor_lane<1>(s);
         if (l == 0 && ok[u]) tail[tt[u]] = search_key(-s, cid[u]);
       }
     }
@@ -1450,13 +1463,19 @@ struct EvalBF16 {
       qs[lane] = f;
     }
   }
+  // the query lives in LDS only
+  __device__ __forceinline__ void load_query_regs(const float* __restrict__,
+                                                  uint32_t) {}
 
+  // Groups of 16 candidates split over `parts` waves: wave `part` takes the
+  // groups g0 = 16G * (part + parts * i).
   __device__ __forceinline__ void eval(const uint32_t* cand,
                                        const uint32_t* todo, uint32_t ntodo,
-                                       uint64_t* tail) const {
+                                       uint64_t* tail, uint32_t part = 0,
+                                       uint32_t parts = 1) const {
     const uint32_t lane = lane_id();
     const uint32_t m = lane & 15u, kq = lane >> 4;
-    for (uint32_t g0 = 0; g0 < ntodo; g0 += 16u * G) {
+    for (uint32_t g0 = 16u * G * part; g0 < ntodo; g0 += 16u * G * parts) {
       bf16x8 a[G][NCH];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
@@ -1526,6 +1545,18 @@ struct EvalFP8 {
   const float* __restrict__ scale8;
   uint32_t dim, dpad, nch;
 
+  // the query's scale (registers): the largest |element| / 448
+  __device__ __forceinline__ void load_query_regs(
+      const float* __restrict__ queries, uint32_t q) {
+    const float* x = queries + size_t(q) * dim;
+    float mx = 0.0f;
+    for (uint32_t e = lane_id(); e < dim; e += kWave)
+      mx = fmaxf(mx, fabsf(x[e]));
+    for (int o = 32; o > 0; o >>= 1)
+      mx = fmaxf(mx, xor_lane_dyn(mx, uint32_t(o)));
+    qscale = mx > 0.0f ? mx / 448.0f : 1.0f;
+  }
+
   __device__ __forceinline__ void load_query(const float* __restrict__ queries,
                                              uint32_t q) {
     const uint32_t lane = lane_id();
@@ -1533,7 +1564,8 @@ struct EvalFP8 {
     float mx = 0.0f;
     for (uint32_t e = lane; e < dim; e += kWave) mx = fmaxf(mx, fabsf(x[e]));
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    for (int o = 32; o > 0; o >>= 1)
+      mx = fmaxf(mx, xor_lane_dyn(mx, uint32_t(o)));
     qscale = mx > 0.0f ? mx / 448.0f : 1.0f;
     const float inv = 1.0f / qscale;
     if (lane < 4u * NCH) {
@@ -1548,10 +1580,11 @@ struct EvalFP8 {
 
   __device__ __forceinline__ void eval(const uint32_t* cand,
                                        const uint32_t* todo, uint32_t ntodo,
-                                       uint64_t* tail) const {
+                                       uint64_t* tail, uint32_t part = 0,
+                                       uint32_t parts = 1) const {
     const uint32_t lane = lane_id();
     const uint32_t m = lane & 15u, kq = lane >> 4;
-    for (uint32_t g0 = 0; g0 < ntodo; g0 += 16u * G) {
+    for (uint32_t g0 = 16u * G * part; g0 < ntodo; g0 += 16u * G * parts) {
       long a[G][NCH];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
@@ -1607,13 +1640,13 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
     mv[r] = idx < m ? tail[todo[idx]] : kPadKey;
   }
   reg_bitonic_sort<R>(mv);
-  __syncthreads();  // ms reuses the candidate / todo slots read above
+  wave_sync();  // ms reuses the candidate / todo slots read above
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t idx = lane * R + r;
     if (idx < m) ms[idx] = mv[r];
   }
-  __syncthreads();
+  wave_sync();
   constexpr int kHpl = HPL;  // Km <= 64 * HPL
   uint64_t hx[kHpl];
   uint32_t hp[kHpl];
@@ -1640,14 +1673,14 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
     }
   }
   if (force_final || __ballot(unexp) == 0) return true;
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int s = 0; s < kHpl; ++s)
     if (hp[s] < Km) keys[hp[s]] = hx[s];
 #pragma unroll
   for (int r = 0; r < R; ++r)
     if (mpos[r] < Km) keys[mpos[r]] = mv[r];
-  __syncthreads();
+  wave_sync();
   return false;
 }
 
@@ -1692,7 +1725,7 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
   }
   if (ne == 0) return (force_final || __ballot(unexp) == 0) ? 1 : 0;
   if (force_final) return 1;
-  __syncthreads();  // ebuf overlays the todo list read above
+  wave_sync();  // ebuf overlays the todo list read above
 #pragma unroll
   for (int c = 0; c < CH; ++c)
     if (in_e[c]) ebuf[slot[c]] = me[c];
@@ -1701,7 +1734,7 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
     const uint32_t p = lane + 64u * s;
     if (p < Km) stage[p] = kPadKey;
   }
-  __syncthreads();
+  wave_sync();
   for (uint32_t e0 = 0; e0 < ne; e0 += kWave) {
     const bool own = e0 + lane < ne;
     const uint64_t e = own ? ebuf[e0 + lane] : kPadKey;
@@ -1713,7 +1746,7 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
       if (r < Km) stage[r] = e;
     }
   }
-  __syncthreads();
+  wave_sync();
   uint64_t v[HPL];
   uint32_t occ_before = 0;
 #pragma unroll
@@ -1726,21 +1759,30 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
     if (p < Km && !occ) v[s] = keys[p - before];
     occ_before += __popcll(om);
   }
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int s = 0; s < HPL; ++s) {
     const uint32_t p = lane + 64u * s;
     if (p < Km) keys[p] = v[s];
   }
-  __syncthreads();
+  wave_sync();
   return 0;
 }
 
 // HPL: top-list slices of 64 keys per lane (Km <= 64*HPL), CH: candidate
 // slices (Kp*Kd <= 64*CH).  Compile-time slice counts keep the merge free of
 // dead slices; larger lists fall back to a full sort per iteration.
+//
+// Waves per query: the workgroup is blockDim.x / 64 waves.  Wave 0 runs the
+// search; any further waves only evaluate distances: at each evaluation
+// wave 0 publishes (command, query, count) in LDS and meets them at a
+// workgroup barrier, every wave evaluates its share of the candidates, and
+// a second barrier hands the keys back.  Between those barriers wave 0
+// orders its own LDS traffic with wave_sync, so the helpers, which wait at
+// the next barrier, never see its other phases.  Command 0 ends the helpers
+// (after wave 0's last query), so every wave passes the same barriers.
 template <class Eval, int HPL, int CH>
-__global__ __launch_bounds__(64)
+__global__ __launch_bounds__(128)
 __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     Eval ev, const float* __restrict__ queries, uint32_t n_query,
     uint32_t n_base,
@@ -1759,9 +1801,29 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
   uint32_t* cand = reinterpret_cast<uint32_t*>(keys + lp2);
   uint32_t* todo = cand + kpkd;
   uint32_t* explore = todo + kpkd;
+  // the evaluation hand-off words: command, query, candidate count
+  volatile uint32_t* hand = explore + Kp;
   // sorted misses of the merge step: reuses cand + todo (2*kpkd words)
   uint64_t* ms = reinterpret_cast<uint64_t*>(cand);
   const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint64_t* tail = keys + Km;
+  if (wave != 0) {
+    uint32_t cur_q = 0xFFFFFFFFu;
+    while (true) {
+      __syncthreads();  // wave 0 has published a command
+      const uint32_t cmd = hand[0];
+      if (cmd == 0) break;
+      const uint32_t qq = hand[1], nt = hand[2];
+      if (qq != cur_q) {
+        ev.load_query_regs(queries, qq);
+        cur_q = qq;
+      }
+      ev.eval(cand, todo, nt, tail, wave, nw);
+      __syncthreads();  // keys handed back
+    }
+    return;
+  }
   // One query per workgroup, or (queue != nullptr) a persistent grid whose
   // waves take query ids from a device counter until it runs past n_query.
   // With a locality order, the queries are taken in that order (nodes
@@ -1798,6 +1860,23 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     return __shfl(v, 0, 64);
   };
   uint32_t q = queue ? next_query() : blockIdx.x;
+  // distances of todo[0, ntodo): all waves of the workgroup take part
+  auto eval_all = [&](uint32_t ntodo) {
+    if (nw > 1) {
+      if (lane == 0) {
+        hand[1] = q;
+        hand[2] = ntodo;
+        hand[0] = 1u;
+      }
+      __syncthreads();
+      ev.eval(cand, todo, ntodo, tail, 0, nw);
+      __syncthreads();
+    } else {
+      wave_sync();
+      ev.eval(cand, todo, ntodo, tail);
+      wave_sync();
+    }
+  };
   while (q < n_query) {
   const bool fast_merge = Km <= 64u * HPL && kpkd <= 64u * CH;
   // the rank merge stages the new top list in the tail region
@@ -1810,11 +1889,8 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     cand[t] = uint32_t((uint64_t(entry_seed) + t) % n_base);
     todo[t] = t;
   }
-  __syncthreads();
-  uint64_t* tail = keys + Km;
-  ev.eval(cand, todo, kpkd, tail);
+  eval_all(kpkd);
   unsigned long long ndist = kpkd;
-  __syncthreads();
   // The list after init is the first Km keys of sort(entry keys, Km tomb
   // keys): the sorted entry keys below the tomb key, then tomb keys.  The
   // tail is rewritten by the first iteration before anything reads it, so
@@ -1835,16 +1911,16 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
       const uint32_t e = lane * RS + r;
       if (e < Km) keys[e] = v[r] < tomb_key ? v[r] : tomb_key;
     }
-    __syncthreads();
+    wave_sync();
   } else {
-    bitonic_sort(keys, lp2);
+    bitonic_sort<uint64_t, true>(keys, lp2);
   }
   auto reset_table = [&]() {
     for (uint32_t i = lane; i < tsize; i += kWave) tab[i] = 0u;
-    __syncthreads();
+    wave_sync();
     for (uint32_t i = lane; i < Km; i += kWave)
       visit_insert(tab, tsize, key_raw_id(keys[i]) & kMaskedTomb);
-    __syncthreads();
+    wave_sync();
   };
   reset_table();
   // Exact evaluators keep the reference's visited set (reset every
@@ -1883,7 +1959,7 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
       break;
     }
     for (uint32_t e = found + lane; e < Kp; e += kWave) explore[e] = kTomb;
-    __syncthreads();
+    wave_sync();
     if (!Eval::kExactVisited && cache_count + kpkd > cache_limit) {
       reset_table();
       cache_count = Km;
@@ -1965,11 +2041,9 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
       if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = t;
       ntodo += __popcll(m);
     }
-    __syncthreads();
-    ev.eval(cand, todo, ntodo, tail);
+    eval_all(ntodo);
     ndist += ntodo;
     cache_count += ntodo;
-    __syncthreads();
     const bool last_allowed = total_iter + 1 >= max_iters;
     bool final_sort = true;
     if (rank_merge) {
@@ -1986,7 +2060,7 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
         final_sort = merge_misses<(CH <= 4 ? 4 : 8), HPL>(
             keys, Km, tail, todo, ntodo, ms, last_allowed);
     }
-    if (final_sort) sort_list(keys, lp2);
+    if (final_sort) sort_list<true>(keys, lp2);
     ++total_iter;
     if (Eval::kExactVisited && ++iter == reset_iter) {
       reset_table();
@@ -2001,9 +2075,13 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     if (capped) stat_add(stats, ST_SEARCH_CAPPED, 1);
   }
   if (queue == nullptr) break;
-  __syncthreads();
+  wave_sync();
   q = next_query();
   }  // query loop
+  if (nw > 1) {
+    if (lane == 0) hand[0] = 0u;
+    __syncthreads();  // the helpers stop
+  }
 }
 
 // ------------------------------------------------------------ launchers
@@ -2039,9 +2117,15 @@ hipError_t launch_match(const uint32_t* gt, uint32_t n_query, uint32_t gt_k,
                         uint32_t n_base, uint32_t* winner, hipStream_t s) {
   hipError_t e = launch_fill_u32(winner, n_base, kTomb, s);
   if (e != hipSuccess) return e;
+  return launch_match_rows(gt, 0, n_query, gt_k, n_base, winner, s);
+}
+
+hipError_t launch_match_rows(const uint32_t* gt, uint32_t q0, uint32_t n_query,
+                             uint32_t gt_k, uint32_t n_base, uint32_t* winner,
+                             hipStream_t s) {
   if (n_query == 0) return hipSuccess;
-  match_winner_kernel<<<grid_for(n_query, 256), 256, 0, s>>>(gt, n_query, gt_k,
-                                                             n_base, winner);
+  match_winner_kernel<<<grid_for(n_query, 256), 256, 0, s>>>(
+      gt, q0, n_query, gt_k, n_base, winner);
   return hipGetLastError();
 }
 
@@ -2193,7 +2277,8 @@ static hipError_t launch_search_eval(
     uint32_t Kd, uint32_t topk, uint32_t reset_iter, uint32_t entry_seed,
     uint32_t max_iters, uint32_t table_size, uint32_t* out,
     unsigned long long* stats, uint32_t* queue, const uint32_t* order,
-    hipStream_t s) {
+    uint32_t waves, hipStream_t s) {
+  const uint32_t block = 64u * (waves == 2 ? 2u : 1u);
   uint32_t lp2 = 64;
   while (lp2 < Km + Kp * Kd) lp2 <<= 1;
   const size_t smem = search_smem_bytes(table_size, lp2, Kp * Kd, Kp) +
@@ -2222,7 +2307,8 @@ static hipError_t launch_search_eval(
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                 dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64, smem);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block,
+                                                       smem);
     if (per_cu < 1) per_cu = 1;
     if (cus < 1) cus = 1;
     const uint64_t resident = uint64_t(per_cu) * uint64_t(cus);
@@ -2231,7 +2317,7 @@ static hipError_t launch_search_eval(
     if (e != hipSuccess) return e;
   }
 #define PG_SEARCH_LAUNCH(H, C)                                              \
-  search_kernel<Eval, H, C><<<grid, 64, smem, s>>>(                         \
+  search_kernel<Eval, H, C><<<grid, block, smem, s>>>(                      \
       ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter, \
       entry_seed, max_iters, table_size, lp2, kd_shift, out, stats, queue,  \
       queue ? order : nullptr)
@@ -2252,13 +2338,14 @@ static hipError_t launch_f32(const float* base, uint32_t dim,
                              uint32_t entry_seed, uint32_t max_iters,
                              uint32_t table_size, uint32_t* out,
                              unsigned long long* stats, uint32_t* queue,
-                             const uint32_t* order, hipStream_t s) {
+                             const uint32_t* order, uint32_t waves,
+                             hipStream_t s) {
   EvalF32<TQ, U> ev;
   ev.base = base;
   ev.dim = dim;
   return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
                             Kd, topk, reset_iter, entry_seed, max_iters,
-                            table_size, out, stats, queue, order, s);
+                            table_size, out, stats, queue, order, waves, s);
 }
 
 template <int NCH, int G>
@@ -2270,7 +2357,8 @@ static hipError_t launch_bf16(const void* base16, uint32_t dim, uint32_t dpad,
                               uint32_t entry_seed, uint32_t max_iters,
                               uint32_t table_size, uint32_t* out,
                               unsigned long long* stats, uint32_t* queue,
-                              const uint32_t* order, hipStream_t s) {
+                              const uint32_t* order, uint32_t waves,
+                              hipStream_t s) {
   EvalBF16<NCH, G> ev;
   ev.base16 = reinterpret_cast<const __bf16*>(base16);
   ev.dim = dim;
@@ -2278,7 +2366,7 @@ static hipError_t launch_bf16(const void* base16, uint32_t dim, uint32_t dpad,
   ev.nch = dpad / 32;
   return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
                             Kd, topk, reset_iter, entry_seed, max_iters,
-                            table_size, out, stats, queue, order, s);
+                            table_size, out, stats, queue, order, waves, s);
 }
 
 template <int NCH, int G>
@@ -2291,7 +2379,7 @@ static hipError_t launch_fp8(const void* base8, const float* scale8,
                              uint32_t max_iters, uint32_t table_size,
                              uint32_t* out, unsigned long long* stats,
                              uint32_t* queue, const uint32_t* order,
-                             hipStream_t s) {
+                             uint32_t waves, hipStream_t s) {
   EvalFP8<NCH, G> ev;
   ev.base8 = reinterpret_cast<const uint8_t*>(base8);
   ev.scale8 = scale8;
@@ -2300,7 +2388,7 @@ static hipError_t launch_fp8(const void* base8, const float* scale8,
   ev.nch = dpad / 32;
   return launch_search_eval(ev, queries, n_query, n_base, graph, gdeg, Km, Kp,
                             Kd, topk, reset_iter, entry_seed, max_iters,
-                            table_size, out, stats, queue, order, s);
+                            table_size, out, stats, queue, order, waves, s);
 }
 
 hipError_t launch_search(const float* queries, uint32_t n_query,
@@ -2313,11 +2401,12 @@ hipError_t launch_search(const float* queries, uint32_t n_query,
                          uint32_t max_iters, uint32_t table_size,
                          uint32_t gather_batch, uint32_t* out,
                          unsigned long long* stats, uint32_t* queue,
-                         const uint32_t* order, hipStream_t s) {
+                         const uint32_t* order, uint32_t waves,
+                         hipStream_t s) {
   if (n_query == 0) return hipSuccess;
 #define PG_ARGS                                                             \
   queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter,      \
-      entry_seed, max_iters, table_size, out, stats, queue, order, s
+      entry_seed, max_iters, table_size, out, stats, queue, order, waves, s
   if (base8 != nullptr) {
     const uint32_t nch = dpad / 32;
     const bool two = gather_batch >= 2;
@@ -2545,7 +2634,8 @@ __global__ __launch_bounds__(256) void prepare_fp8_kernel(
     float mx = 0.0f;
     for (uint32_t c = lane; c < dim; c += 64u) mx = fmaxf(mx, fabsf(x[c]));
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    for (int o = 32; o > 0; o >>= 1)
+      mx = fmaxf(mx, xor_lane_dyn(mx, uint32_t(o)));
     const float sc = mx > 0.0f ? mx / 448.0f : 1.0f;
     const float inv = 1.0f / sc;
     // 8 consecutive bytes per lane per step

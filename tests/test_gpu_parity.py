@@ -159,6 +159,38 @@ def test_whole_stage_matches_oracle(name, cuda):
         assert st[k] == so[k], (k, st[k], so[k])
 
 
+def test_cfg1_100k_whole_stage_matches_oracle(cuda):
+    """The 100K x 200 configuration at its stated size (64-component
+    mixture, rotated + offset training queries, exact top-128 inner-product
+    gt, M 32, L 100): the whole GPU stage, with the library's default
+    knobs and with the locality-ordered queue, equals the oracle's
+    projection and link rows id for id, counters included."""
+    import torch
+    w = datagen.make_config("cfg1_100k_d200", device="cuda")
+    base = w["base"]
+    n, dim = base.shape
+    gt = np.ascontiguousarray(w["gt"].view(np.uint32))
+    print(f"[parity] cfg1 whole stage n={n} dim={dim}", flush=True)
+    p = _native.make_params(n, dim, n, w["M"], w["L"])
+    st_o = cpu_ref.OrcStats()
+    proj_o, link_o = cpu_ref.gpu_stage(orc_params(p), base, gt, stats=st_o)
+    base_d = to_dev(base, torch)
+    gt_d = to_dev(gt, torch)
+    for flags in (p.tune_flags, 0x624):
+        pp = _native.PgParams.from_buffer_copy(p)
+        pp.tune_flags = flags
+        proj_d = torch.empty((n, pp.proj_deg), dtype=torch.int32, device=cuda)
+        link_d = torch.empty((n, pp.M), dtype=torch.int32, device=cuda)
+        with _native.Context(pp) as ctx:
+            ctx.stage_device(base_d, gt_d, proj_d, link_d)
+            ctx.wait()
+            st = ctx.stats().as_dict()
+        np.testing.assert_array_equal(from_dev(proj_d), proj_o)
+        np.testing.assert_array_equal(from_dev(link_d), link_o)
+        for k, v in st_o.as_dict().items():
+            assert st[k] == v, (flags, k, st[k], v)
+
+
 def test_host_entry_point_matches_device_entry_point(cuda):
     import torch
     w = workload("d64_m32_l64")
@@ -176,6 +208,66 @@ def test_host_entry_point_matches_device_entry_point(cuda):
         ctx.wait()
     np.testing.assert_array_equal(proj_h, from_dev(proj_d))
     np.testing.assert_array_equal(link_h, from_dev(link_d))
+
+
+@pytest.mark.parametrize("prec,flags", [
+    (_native.PG_FP32, 0x20), (_native.PG_BF16, 0x221),
+    (_native.PG_FP8E4M3, 0x20), (_native.PG_BF16, 0x60)],
+    ids=["fp32", "bf16-order", "fp8", "bf16-graph"])
+@pytest.mark.parametrize("pinned", [True, False], ids=["pinned", "pageable"])
+def test_streamed_host_entry_matches_device_entry(prec, flags, pinned, cuda):
+    """The host entry point streams its inputs in several chunks (gt chunks
+    matched and base chunks screen-converted as they land) and its outputs
+    out: the rows equal the device entry's, on two builds in a row from the
+    same context (the second must wait for the first), from pinned
+    (pg_host_alloc) and pageable buffers, and in graph mode."""
+    import torch
+    w = workload("d200_m32_l100")
+    p = _native.PgParams.from_buffer_copy(w["params"])
+    p.precision = prec
+    p.tune_flags = flags
+    n = p.n_base
+    print(f"streamed host entry prec={prec} flags={flags:#x} "
+          f"pinned={pinned}", flush=True)
+    keep = []
+
+    def buf(shape, dtype, src=None):
+        if pinned:
+            pa = _native.PinnedArray(shape, dtype)
+            keep.append(pa)
+            a = pa.a
+        else:
+            a = np.empty(shape, dtype)
+        if src is not None:
+            a[:] = src
+        return a
+
+    base_h = buf(w["base"].shape, np.float32, w["base"])
+    gt_h = buf(w["gt_u32"].shape, np.uint32, w["gt_u32"])
+    with _native.Context(p) as ctx:
+        proj_d = torch.empty((n, p.proj_deg), dtype=torch.int32, device=cuda)
+        link_d = torch.empty((n, p.M), dtype=torch.int32, device=cuda)
+        ctx.stage_device(to_dev(w["base"], torch), to_dev(w["gt_u32"], torch),
+                         proj_d, link_d)
+        ctx.wait()
+        for rep in range(2):
+            proj_h = buf((n, p.proj_deg), np.uint32)
+            link_h = buf((n, p.M), np.uint32)
+            proj_h[:] = 7
+            link_h[:] = 7
+            ctx.stage_host(base_h, gt_h, proj_h, link_h)
+            if rep == 0:
+                # a second build enqueued behind the first one
+                proj_h2 = buf((n, p.proj_deg), np.uint32)
+                link_h2 = buf((n, p.M), np.uint32)
+                ctx.stage_host(base_h, gt_h, proj_h2, link_h2)
+            ctx.wait()
+            np.testing.assert_array_equal(proj_h, from_dev(proj_d))
+            np.testing.assert_array_equal(link_h, from_dev(link_d))
+            if rep == 0:
+                np.testing.assert_array_equal(proj_h2, from_dev(proj_d))
+                np.testing.assert_array_equal(link_h2, from_dev(link_d))
+    del keep
 
 
 def test_table_size_changes_nothing(cuda):
@@ -241,6 +333,47 @@ def test_persistent_search_matches_oracle(cuda):
     ref = cpu_ref.search(w["base"], w["base"], graph, p.Km, p.Kp, p.Kd,
                          p.topk)
     np.testing.assert_array_equal(from_dev(out_d), ref)
+
+
+WAVE_CASES = [(prec, flags, L)
+              for prec in (_native.PG_FP32, _native.PG_BF16,
+                           _native.PG_FP8E4M3)
+              for flags in (0x800, 0x820, 0xA20, 0x802)
+              for L in (48, 100)]
+
+
+@pytest.mark.parametrize("prec,flags,L", WAVE_CASES,
+                         ids=[f"p{c[0]}-{c[1]:#x}-L{c[2]}" for c in WAVE_CASES])
+def test_two_waves_per_query_same_rows(prec, flags, L, cuda):
+    """Two waves per query (tune bit 11: the second wave shares the
+    distance evaluations) give the same search rows and counters as one
+    wave, with one workgroup per query, the persistent grid, the locality
+    order and a wider gather; FP32 rows also equal the oracle's."""
+    import torch
+    w = workload("d200_m32_l100")
+    n = w["params"].n_base
+    rng = np.random.default_rng(7)
+    graph = rng.integers(0, n, (n, 16)).astype(np.uint32)
+    graph[rng.random((n, 16)) < 0.1] = 0xFFFFFFFF
+    base_d, graph_d = to_dev(w["base"], torch), to_dev(graph, torch)
+    print(f"two waves prec={prec} flags={flags:#x} L={L}", flush=True)
+    outs, stats = [], []
+    for f in (flags & ~0x800, flags):
+        p = _native.make_params(n, w["base"].shape[1], n, 32, L,
+                                precision=prec, tune_flags=f)
+        out_d = torch.full((n, p.topk), -7, dtype=torch.int32, device=cuda)
+        with _native.Context(p) as ctx:
+            ctx.run_search(base_d, base_d, graph_d, out_d)
+            ctx.sync()
+            st = ctx.stats()
+        outs.append(from_dev(out_d))
+        stats.append((st.search_dist, st.search_iters))
+    np.testing.assert_array_equal(outs[1], outs[0])
+    assert stats[1] == stats[0]
+    if prec == _native.PG_FP32:
+        ref = cpu_ref.search(w["base"], w["base"], graph, p.Km, p.Kp, p.Kd,
+                             p.topk)
+        np.testing.assert_array_equal(outs[1], ref)
 
 
 @pytest.mark.parametrize("n_query", [2000, 1003, 5, 1])
