@@ -1,22 +1,23 @@
-"""The large configurations (10M x 200; 5M x 512 with Zipf queries and
-ragged gt) through size-independent checks: the stage-by-stage pipeline
-through the C ABI, with the oracle re-running search, prune and merge on a
-random node sample from the same inputs (FP32 rows must match id-for-id);
-the whole-stage entry equal to the stage-by-stage composition; determinism;
-row invariants.  The 1M x 200 and 1M x 512 builds also run at full size
-with their tuned settings: FP32 at 1M x 512 (persistent locality-ordered
-queue, merge in the same order), the BF16-screened path at both (beam
-56 and 64, visited caches of 960 and 1024 slots; at 1M x 512 the split
-screen and the candidate-major prune), and the FP8-screened path at
-1M x 200 (beam 48, visited cache of 768 slots).  A screened search is not the
-FP32 one, so there only the match, prune, reverse CSR and merge, which
-decide every edge at FP32, are checked against the oracle on the sample.
+"""Every benchmark configuration at its stated size, through
+size-independent checks: the stage-by-stage pipeline through the C ABI,
+with the oracle re-running the match (in full) and the prune, reverse CSR,
+merge and (FP32) search on a random node sample from the same inputs (rows
+must match id-for-id); the whole-stage entry equal to the stage-by-stage
+composition; determinism; row invariants.
 
-Default sizes are scaled down (1M and 100K base rows, exact gt).  With
-PG_SCALE_FULL=1 the configurations run at their full sizes; gt is then the
-exact top-128 within a fixed random 200K-row subset of the base (an N x N
-exact gt is out of reach for a test, and any well-formed gt exercises the
-build the same way)."""
+  10M x 200 (cfg4) and 5M x 512 Zipf with ragged gt (cfg5): FP32 and the
+  BF16-screened build;
+  1M x 200 (cfg2): the BF16 and FP8 value paths (tuned beams and knobs);
+  1M x 512 (cfg3): FP32 (locality-ordered queue and merge), BF16 (split
+  screen, candidate-major prune) and the FP8 e4m3 leg.
+
+A screened search is not the FP32 one, so on the screened legs the oracle
+checks the match, prune, reverse CSR and merge rows (every edge is decided
+at FP32 there).  The 10M and 5M builds take their gt from 200K-row shards
+(the exact top-128 within the shard a query is assigned to): an exact N x N
+gt at those sizes is a bench input, and any well-formed gt exercises the
+build the same way.  PG_SCALE_SMALL=1 runs the two large configurations
+scaled to 1M and 100K rows with exact gt (a quick run)."""
 import os
 
 import numpy as np
@@ -27,20 +28,42 @@ from paragraph_amd import _native, datagen
 
 pytestmark = pytest.mark.gpu
 TOMB = 0xFFFFFFFF
-FULL = os.environ.get("PG_SCALE_FULL") == "1"
-# (config, scale, precision, parameter overrides, beam)
-CASES = [("cfg4_10m_d200", 1.0 if FULL else 0.1, "fp32", {}, None),
-         ("cfg5_5m_d512_zipf", 1.0 if FULL else 0.02, "fp32", {}, None),
-         ("cfg2_1m_d200", 1.0, "bf16",
-          dict(precision=_native.PG_BF16, tune_flags=0x221, table_size=960),
-          56),
-         ("cfg2_1m_d200", 1.0, "fp8",
-          dict(precision=_native.PG_FP8E4M3, tune_flags=0x221,
-               table_size=768), 48),
-         ("cfg3_1m_d512", 1.0, "fp32", dict(tune_flags=0x624), None),
-         ("cfg3_1m_d512", 1.0, "bf16",
-          dict(precision=_native.PG_BF16, tune_flags=0x7A1, table_size=1024),
-          64)]
+SMALL = os.environ.get("PG_SCALE_SMALL") == "1"
+LARGE_SCALE = {"cfg4_10m_d200": 0.1 if SMALL else 1.0,
+               "cfg5_5m_d512_zipf": 0.02 if SMALL else 1.0}
+BF16 = _native.PG_BF16
+FP8 = _native.PG_FP8E4M3
+# (config, precision, parameter overrides, beam)
+CASES = [("cfg4_10m_d200", "fp32", {}, None),
+         ("cfg4_10m_d200", "bf16",
+          dict(precision=BF16, tune_flags=0x221, table_size=768), 48),
+         ("cfg5_5m_d512_zipf", "fp32", {}, None),
+         ("cfg5_5m_d512_zipf", "bf16", dict(precision=BF16, tune_flags=0x221),
+          None),
+         ("cfg2_1m_d200", "bf16",
+          dict(precision=BF16, tune_flags=0x221, table_size=960), 56),
+         ("cfg2_1m_d200", "fp8",
+          dict(precision=FP8, tune_flags=0x221, table_size=768), 48),
+         ("cfg3_1m_d512", "fp32", dict(tune_flags=0x624), None),
+         ("cfg3_1m_d512", "bf16",
+          dict(precision=BF16, tune_flags=0x7A1, table_size=1024), 64),
+         ("cfg3_1m_d512", "fp8",
+          dict(precision=FP8, tune_flags=0x221, table_size=1024), 64)]
+
+_workloads = {}
+
+
+def _workload(cfg):
+    """One generated workload per configuration, shared by its legs (the
+    cases of a configuration are adjacent, so one is kept at a time)."""
+    if cfg not in _workloads:
+        _workloads.clear()
+        scale = LARGE_SCALE.get(cfg, 1.0)
+        large = cfg in LARGE_SCALE and not SMALL
+        _workloads[cfg] = datagen.make_config(
+            cfg, scale=scale, device="cuda",
+            gt_subset=200_000 if large else None)
+    return _workloads[cfg]
 
 
 def _u(t):
@@ -54,12 +77,17 @@ def _sorted_segments(off, srcs):
     return srcs[np.lexsort((srcs, seg))]
 
 
-@pytest.mark.parametrize("cfg,scale,prec,over,beam", CASES,
-                         ids=[f"{c[0]}-{c[2]}" for c in CASES])
-def test_scale_sampled_parity(cfg, scale, prec, over, beam, cuda):
+@pytest.mark.parametrize("cfg,prec,over,beam", CASES,
+                         ids=[f"{c[0]}-{c[1]}" for c in CASES])
+def test_scale_sampled_parity(cfg, prec, over, beam, cuda):
+    import time
+
     import torch
-    w = datagen.make_config(cfg, scale=scale, device="cuda",
-                            gt_subset=200_000 if FULL else None)
+    t0 = time.time()
+    print(f"[scale] {cfg} {prec}: generating", flush=True)
+    w = _workload(cfg)
+    print(f"[scale] {cfg} {prec}: workload ready after {time.time() - t0:.0f}"
+          " s", flush=True)
     base = w["base"]
     n, dim = base.shape
     M, L = w["M"], beam or w["L"]
@@ -102,6 +130,8 @@ def test_scale_sampled_parity(cfg, scale, prec, over, beam, cuda):
         om = cpu_ref.merge_nodes(base, S, ooff, osrcs, p.proj_rev, pr, pc,
                                  p.proj_deg, False)
         assert np.array_equal(_u(graph_d)[S], om)
+        print(f"[scale] {cfg} {prec}: projection checked after "
+              f"{time.time() - t0:.0f} s", flush=True)
         for r in range(p.rounds):
             graph = _u(graph_d)
             res_d = torch.empty((n, p.topk), **i32)
@@ -135,6 +165,8 @@ def test_scale_sampled_parity(cfg, scale, prec, over, beam, cuda):
                                      p.M, True)
             assert np.array_equal(_u(link_d)[S], om), f"merge round {r + 1}"
             graph_d = link_d
+            print(f"[scale] {cfg} {prec}: round {r + 1} checked after "
+                  f"{time.time() - t0:.0f} s", flush=True)
         # whole stage: equal to the composition, and deterministic
         proj_d = torch.empty((n, p.proj_deg), **i32)
         link2_d = torch.empty((n, p.M), **i32)
