@@ -101,6 +101,25 @@ int pg_host_fusion_final(const pg_host_params* p, const float* base,
 int pg_host_save_index(const char* path, uint32_t ep, uint32_t n,
                        const uint32_t* off, const uint32_t* ids);
 
+/* Vector and gt files.  Big-ANN .fbin / .ibin: uint32 rows, uint32 cols,
+ * then rows x cols 4-byte elements (float32 / int32), row-major (README.md:
+ * 15-21); the reference's loader reads the same arrays with no header
+ * (include/utils.hpp:18-34, after transform_data_to_gpu.ipynb strips it).
+ * pg_host_bin_shape reads the header.  pg_host_read_bin reads rows
+ * [row0, row0 + nrows) into out (nrows x cols elements), so a large file
+ * can be streamed chunk by chunk into pinned staging memory
+ * (pg_host_alloc, paragraph.h); headered = 0 reads a headerless file of
+ * `cols` columns, headered = 1 checks cols against the header (0 = take
+ * the header's). */
+int pg_host_bin_shape(const char* path, uint32_t* rows, uint32_t* cols);
+int pg_host_read_bin(const char* path, int headered, uint32_t cols,
+                     uint64_t row0, uint64_t nrows, void* out);
+
+/* gt rows re-padded (or truncated) to out_cols columns with -1, as
+ * transform_gt_to_128.ipynb pads the reference's gt to 128. */
+int pg_host_pad_gt(const int32_t* in, uint64_t rows, uint32_t in_cols,
+                   uint32_t out_cols, int32_t* out);
+
 const char* pg_host_last_error(void);
 
 #ifdef __cplusplus

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libparagraph_hip.so")
 ARCH = os.environ.get("PG_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pg_kernels.hip", "pg_capi.cpp"]
+HIP_SOURCES = ["pg_kernels.hip", "pg_topk.hip", "pg_capi.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -80,7 +80,31 @@ def build_host(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("host stage build failed")
     os.replace(tmp, HOST_LIB)
+    build_tool(verbose)
     return HOST_LIB
+
+
+TOOL = os.path.join(HERE, "pg_build_index")
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+
+
+def build_tool(verbose=False):
+    """pg_build_index, the command-line whole build on the C++ front end
+    (include/paragraph/Builder.hpp), linked against both libraries."""
+    cxx = shutil.which("g++") or "g++"
+    tmp = TOOL + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-pthread",
+           os.path.join(CSRC, "pg_build_index.cpp"), "-o", tmp,
+           "-L" + HERE, "-lparagraph_hip", "-lparagraph_host",
+           "-L" + ROCM_LIB, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + ROCM_LIB]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("pg_build_index build failed")
+    os.replace(tmp, TOOL)
+    return TOOL
 
 
 if __name__ == "__main__":

@@ -21,6 +21,20 @@ import numpy as np
 from . import _native, host_stage
 
 
+def resolve_host_stages(host_stages, n_base, n_query):
+    """Where BuildIndex runs the host stages: "gpu" needs n_query ==
+    n_base (pg_run_fusion indexes its supply lists by base row)."""
+    if host_stages == "auto":
+        return "gpu" if n_query == n_base else "host"
+    if host_stages not in ("gpu", "host"):
+        raise ValueError('host_stages must be "auto", "gpu" or "host"')
+    if host_stages == "gpu" and n_query != n_base:
+        raise ValueError("host_stages='gpu' needs one gt row per base row "
+                         f"({n_query} gt rows, {n_base} base rows); use "
+                         "'host' or 'auto'")
+    return host_stages
+
+
 class Builder:
     def __init__(self, base, gt_ids, dim=None):
         base = np.ascontiguousarray(base, dtype=np.float32)
@@ -62,22 +76,24 @@ class Builder:
 
     def BuildIndex(self, M=32, L=128, path=None, device=0,
                    precision=_native.PG_FP32, host_params=None,
-                   host_stages="gpu", **overrides):
+                   host_stages="auto", **overrides):
         """Returns (ep, off, ids): the final lists as CSR.  host_params:
         host_stage.make_params overrides (M_nn, rounds, M_final, threads).
         host_stages="gpu" runs MatchSup / FusionNN / FusionFinal on the GPU
-        after the GPU stage (pg_run_fusion); "host" runs them on the host
-        cores, the bipartite part beside the GPU stage as in the reference.
-        Both give the same lists."""
+        after the GPU stage (pg_run_fusion), which needs one gt row per
+        base row; "host" runs them on the host cores, the bipartite part
+        beside the GPU stage as in the reference, for any gt row count;
+        "auto" (the default) takes "gpu" when the gt has one row per base
+        row and "host" otherwise.  Both give the same lists."""
         n, dim = self.base_.shape
         hp = host_stage.make_params(n, dim, self.gt_ids_.shape[0],
                                     gt_k=self.gt_ids_.shape[1],
                                     **(host_params or {}))
+        host_stages = resolve_host_stages(host_stages, n,
+                                          self.gt_ids_.shape[0])
         if host_stages == "gpu":
             return self._build_index_gpu(hp, M, L, path, device, precision,
                                          overrides)
-        if host_stages != "host":
-            raise ValueError('host_stages must be "gpu" or "host"')
         bip = {}
 
         def host_work():

@@ -445,4 +445,81 @@ int pg_host_save_index(const char* path, uint32_t ep, uint32_t n,
   return ok ? PG_OK : fail(PG_ERR_INVALID, "write to %s failed", path);
 }
 
+int pg_host_bin_shape(const char* path, uint32_t* rows, uint32_t* cols) {
+  if (!path || !rows || !cols) return fail(PG_ERR_INVALID, "NULL argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(PG_ERR_INVALID, "cannot open %s", path);
+  uint32_t hdr[2];
+  const bool ok = std::fread(hdr, 4, 2, f) == 2;
+  std::fclose(f);
+  if (!ok) return fail(PG_ERR_INVALID, "%s: no 8-byte header", path);
+  *rows = hdr[0];
+  *cols = hdr[1];
+  return PG_OK;
+}
+
+int pg_host_read_bin(const char* path, int headered, uint32_t cols,
+                     uint64_t row0, uint64_t nrows, void* out) {
+  if (!path || (nrows && !out)) return fail(PG_ERR_INVALID, "NULL argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(PG_ERR_INVALID, "cannot open %s", path);
+  uint64_t file_rows = 0;
+  long long base_off = 0;
+  if (headered) {
+    uint32_t hdr[2];
+    if (std::fread(hdr, 4, 2, f) != 2) {
+      std::fclose(f);
+      return fail(PG_ERR_INVALID, "%s: no 8-byte header", path);
+    }
+    if (cols != 0 && cols != hdr[1]) {
+      std::fclose(f);
+      return fail(PG_ERR_INVALID, "%s has %u columns, not %u", path, hdr[1],
+                  cols);
+    }
+    cols = hdr[1];
+    file_rows = hdr[0];
+    base_off = 8;
+  } else {
+    if (cols == 0) {
+      std::fclose(f);
+      return fail(PG_ERR_INVALID, "a headerless file needs cols");
+    }
+    if (std::fseek(f, 0, SEEK_END) != 0) {
+      std::fclose(f);
+      return fail(PG_ERR_INVALID, "cannot seek in %s", path);
+    }
+    file_rows = uint64_t(std::ftell(f)) / (4ull * cols);
+  }
+  if (row0 + nrows > file_rows) {
+    std::fclose(f);
+    return fail(PG_ERR_INVALID, "%s: rows [%llu, %llu) beyond its %llu rows",
+                path, (unsigned long long)row0,
+                (unsigned long long)(row0 + nrows),
+                (unsigned long long)file_rows);
+  }
+  const uint64_t elems = nrows * cols;
+  bool ok = std::fseek(f, long(base_off + row0 * 4ull * cols), SEEK_SET) == 0;
+  // in pieces of 64 MiB (fread counts are size_t)
+  uint8_t* dst = static_cast<uint8_t*>(out);
+  for (uint64_t done = 0; ok && done < elems;) {
+    const uint64_t take = std::min<uint64_t>(elems - done, 16ull << 20);
+    ok = std::fread(dst + 4 * done, 4, size_t(take), f) == take;
+    done += take;
+  }
+  std::fclose(f);
+  return ok ? PG_OK : fail(PG_ERR_INVALID, "short read from %s", path);
+}
+
+int pg_host_pad_gt(const int32_t* in, uint64_t rows, uint32_t in_cols,
+                   uint32_t out_cols, int32_t* out) {
+  if ((rows && (!in || !out)) || out_cols == 0)
+    return fail(PG_ERR_INVALID, "NULL argument or zero width");
+  const uint32_t k = in_cols < out_cols ? in_cols : out_cols;
+  for (uint64_t r = 0; r < rows; ++r) {
+    std::memcpy(out + r * out_cols, in + r * in_cols, 4ull * k);
+    for (uint32_t c = k; c < out_cols; ++c) out[r * out_cols + c] = -1;
+  }
+  return PG_OK;
+}
+
 }  // extern "C"

@@ -20,7 +20,8 @@ _U32 = ctypes.c_uint32
 EXPORTED = (
     "pg_host_params_init", "pg_host_entry_point", "pg_host_match_sup",
     "pg_host_bipartite", "pg_host_final_width", "pg_host_fusion_final",
-    "pg_host_save_index", "pg_host_last_error",
+    "pg_host_save_index", "pg_host_bin_shape", "pg_host_read_bin",
+    "pg_host_pad_gt", "pg_host_last_error",
 )
 
 
@@ -54,6 +55,11 @@ def lib():
         L.pg_host_fusion_final.argtypes = [ctypes.POINTER(HostParams), _P, _P,
                                            _P, _P, _U32, _P, _U32, _P, _P]
         L.pg_host_save_index.argtypes = [ctypes.c_char_p, _U32, _U32, _P, _P]
+        L.pg_host_bin_shape.argtypes = [ctypes.c_char_p, ctypes.POINTER(_U32),
+                                        ctypes.POINTER(_U32)]
+        L.pg_host_read_bin.argtypes = [ctypes.c_char_p, ctypes.c_int, _U32,
+                                       ctypes.c_uint64, ctypes.c_uint64, _P]
+        L.pg_host_pad_gt.argtypes = [_P, ctypes.c_uint64, _U32, _U32, _P]
         L.pg_host_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -134,6 +140,50 @@ def save_index(path, ep, off, ids):
     off, ids = _u32(off), _u32(ids)
     _check(lib().pg_host_save_index(os.fsencode(path), ep, off.shape[0] - 1,
                                     _ptr(off), _ptr(ids)))
+
+
+def bin_shape(path):
+    """(rows, cols) from a Big-ANN .fbin / .ibin header."""
+    r, c = _U32(), _U32()
+    _check(lib().pg_host_bin_shape(os.fsencode(path), ctypes.byref(r),
+                                   ctypes.byref(c)))
+    return int(r.value), int(c.value)
+
+
+def read_bin(path, dtype, headered=True, cols=0, row0=0, nrows=None,
+             out=None):
+    """Rows [row0, row0 + nrows) of a .fbin / .ibin file (headered) or of a
+    headerless array of `cols` columns, read by the C++ reader into `out`
+    (a C-contiguous array, e.g. pinned staging memory) or a new array."""
+    dtype = np.dtype(dtype)
+    if dtype.itemsize != 4:
+        raise ValueError("4-byte elements (float32 / int32 / uint32)")
+    if headered:
+        rows, cols_h = bin_shape(path)
+        cols = cols or cols_h
+    else:
+        if not cols:
+            raise ValueError("a headerless file needs cols")
+        rows = os.path.getsize(path) // (4 * cols)
+    if nrows is None:
+        nrows = rows - row0
+    if out is None:
+        out = np.empty((nrows, cols), dtype)
+    if out.size != nrows * cols or not out.flags.c_contiguous:
+        raise ValueError("out must be C-contiguous with nrows x cols elements")
+    _check(lib().pg_host_read_bin(os.fsencode(path), 1 if headered else 0,
+                                  cols, row0, nrows, _ptr(out)))
+    return out
+
+
+def pad_gt(gt, width=128):
+    """gt rows padded (or truncated) to `width` columns with -1 (int32), by
+    the C++ routine."""
+    g = np.ascontiguousarray(gt, dtype=np.int32)
+    out = np.empty((g.shape[0], width), np.int32)
+    _check(lib().pg_host_pad_gt(_ptr(g), g.shape[0], g.shape[1], width,
+                                _ptr(out)))
+    return out
 
 
 def default_threads():

@@ -106,6 +106,19 @@ def test_autotune_candidates_valid():
         assert pp.tune_flags == 0x21
 
 
+def test_builder_host_stages_choice():
+    """BuildIndex runs the host stages on the GPU only when the gt has one
+    row per base row; "auto" falls back to the host stages otherwise."""
+    from paragraph_amd.builder import resolve_host_stages
+    assert resolve_host_stages("auto", 100, 100) == "gpu"
+    assert resolve_host_stages("auto", 100, 60) == "host"
+    assert resolve_host_stages("host", 100, 100) == "host"
+    with pytest.raises(ValueError):
+        resolve_host_stages("gpu", 100, 60)
+    with pytest.raises(ValueError):
+        resolve_host_stages("cpu", 100, 100)
+
+
 def test_builder_rejects_unknown_precision():
     from paragraph_amd import Builder
     b = Builder(np.zeros((4, 8), np.float32), np.zeros((4, 128), np.int32))

@@ -91,3 +91,40 @@ def test_eval_search_over_final_index_matches_oracle(cuda):
     ref = cpu_ref.search(q, g["base"], rows, p.Km, p.Kp, p.Kd, p.topk,
                          reset_iter=p.reset_iter)
     np.testing.assert_array_equal(out_d.cpu().numpy().view(np.uint32), ref)
+
+
+def test_cpp_front_end_build_tool(cuda, tmp_path):
+    """The C++ front end run for real: pg_build_index (Gbuilder::Builder
+    LaunchBuild + BuildIndex + SaveIndex, reading .fbin / .ibin with the
+    C++ reader, gt of 100 columns re-padded to 128) gives the golden rows
+    and the oracle's final index, id for id."""
+    import subprocess
+    from oracle import cpu_ref
+    from paragraph_amd import build, io
+    g = golden_io.load_stage("stage_d200")
+    pr = g["params"]
+    gt_i = g["gt"].view(np.int32)
+    fb, fg = str(tmp_path / "base.fbin"), str(tmp_path / "gt.ibin")
+    io.write_bin(fb, g["base"])
+    # 100 stored columns; the tool pads to 128 with -1 like the notebook
+    io.write_bin(fg, np.ascontiguousarray(gt_i[:, :100]))
+    fi_, fl, fp = (str(tmp_path / x) for x in ("index.bin", "link.ibin",
+                                               "proj.ibin"))
+    print(f"pg_build_index n={pr['n_base']} M={pr['M']} L={pr['Km']}",
+          flush=True)
+    r = subprocess.run([build.TOOL, fb, fg, str(pr["M"]), str(pr["Km"]), fi_,
+                        fl, fp], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    gt_pad = io.pad_gt(gt_i[:, :100]).view(np.uint32)
+    # the golden rows were built from the full 128-column gt: compare with
+    # the Python front end on the same padded gt instead
+    b = Builder(g["base"], gt_pad.view(np.int32))
+    proj, link = b.LaunchBuild(M=pr["M"], L=pr["Km"])
+    np.testing.assert_array_equal(io.read_bin(fl, np.uint32), link)
+    np.testing.assert_array_equal(io.read_bin(fp, np.uint32), proj)
+    bo, bi = cpu_ref.bipartite(g["base"], gt_pad, 40, 6)
+    fo, fi = cpu_ref.fusion_final(g["base"], bo, bi, link, proj, 70)
+    ep, lists = io.load_index(fi_)
+    assert ep == cpu_ref.entry_point(gt_pad, pr["n_base"])
+    assert all(np.array_equal(lists[i], fi[fo[i]:fo[i + 1]])
+               for i in range(pr["n_base"]))

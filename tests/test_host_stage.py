@@ -406,6 +406,48 @@ def test_index_file_layout(tmp_path):
     assert ep == 1 and [l.tolist() for l in lists] == [[1, 2], [], [0, 1, 2]]
 
 
+def test_bin_reader_headered_headerless_and_chunks(tmp_path):
+    """The C++ .fbin / .ibin reader against numpy: whole files, row ranges
+    (streaming chunks), headerless arrays, and its size checks."""
+    rng = np.random.default_rng(14)
+    a = rng.standard_normal((37, 5)).astype(np.float32)
+    g = rng.integers(-1, 1000, (37, 9)).astype(np.int32)
+    fa, fg, raw = (str(tmp_path / x) for x in ("a.fbin", "g.ibin", "a.raw"))
+    io.write_bin(fa, a)
+    io.write_bin(fg, g)
+    io.write_bin(raw, a, headered=False)
+    assert host_stage.bin_shape(fa) == (37, 5)
+    np.testing.assert_array_equal(host_stage.read_bin(fa, np.float32), a)
+    np.testing.assert_array_equal(host_stage.read_bin(fg, np.int32), g)
+    parts = [host_stage.read_bin(fa, np.float32, row0=r, nrows=min(8, 37 - r))
+             for r in range(0, 37, 8)]
+    np.testing.assert_array_equal(np.concatenate(parts), a)
+    np.testing.assert_array_equal(
+        host_stage.read_bin(raw, np.float32, headered=False, cols=5,
+                            row0=30, nrows=7), a[30:])
+    np.testing.assert_array_equal(io.read_bin(fa, np.float32), a)
+    with pytest.raises(host_stage.HostError, match="beyond"):
+        host_stage.read_bin(fa, np.float32, row0=30, nrows=8)
+    with pytest.raises(host_stage.HostError, match="columns"):
+        host_stage.read_bin(fa, np.float32, cols=4, nrows=1)
+    for width in (128, 9, 4):
+        np.testing.assert_array_equal(host_stage.pad_gt(g, width),
+                                      io.pad_gt(g, width))
+
+
+def test_build_tool_usage():
+    """pg_build_index (the C++ front end's command-line build) rejects a
+    bad command line before touching a GPU."""
+    import os
+    import subprocess
+    from paragraph_amd import build
+    if not os.path.exists(build.TOOL):
+        pytest.skip("pg_build_index is not built")
+    r = subprocess.run([build.TOOL, "only-one-arg"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
 def test_host_library_exports_and_errors():
     L = host_stage.lib()
     for name in host_stage.EXPORTED:
