@@ -197,8 +197,8 @@ def main():
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--no-dist-error", action="store_true")
     ap.add_argument("--no-host-window", action="store_true",
-                    help="skip the one build from host buffers (PCIe-"
-                         "inclusive rate, reported beside value)")
+                    help="skip the builds from host buffers (PCIe-"
+                         "inclusive rates, reported beside value)")
     ap.add_argument("--n-eval", type=int, default=10000)
     ap.add_argument("--tune-flags", type=int, default=None,
                     help="perf-only knobs for the FP32 build (default: the "
@@ -430,27 +430,15 @@ def main():
                                  args.steps, eval_ctx, p_eval, eq_d, truth,
                                  final_index, log)
 
-    # ---- host window: one build of the value path from pageable host
-    # buffers through the host entry point (H2D + stage + D2H); reported
-    # beside `value`, never as it
+    # ---- host window: whole builds from host buffers through the host
+    # entry point (inputs streamed in on the copy stream, outputs streamed
+    # out), for every precision: pinned buffers (pg_host_alloc), timed over
+    # the same steps, and one build from pageable buffers; reported beside
+    # `value`, never as it
     host_window = None
     if not args.no_host_window:
-        hp = results[[x for x in precs if x != "fp32"][0] if any(
-            x != "fp32" for x in precs) else "fp32"]
-        hctx = hp["ctx"]
-        hproj = np.empty((n, hctx.params.proj_deg), np.uint32)
-        hlink = np.empty((n, M), np.uint32)
-        base_h = np.ascontiguousarray(w["base"])
-        torch.cuda.synchronize()
-        t_h = time.perf_counter()
-        hctx.stage_host(base_h, gt_u32, hproj, hlink)
-        hctx.wait()
-        ms_h = 1000.0 * (time.perf_counter() - t_h)
-        host_window = {"ms": round(ms_h, 2),
-                       "base_points_per_s": round(n / (ms_h / 1000.0), 1),
-                       "same_rows_as_device_entry": bool(np.array_equal(
-                           hlink, hp["link"].cpu().numpy().view(np.uint32)))}
-        log(f"host window (H2D + stage + D2H): {host_window}")
+        host_window = time_host_window(results, w, gt_u32, n, M,
+                                       max(1, args.steps), log)
 
     # ---- end-to-end window: the reference's whole build
     # (tests/test_t2iend.cu:850-1243), inputs in host memory for the host
@@ -560,6 +548,54 @@ def main():
     print(json.dumps(line), flush=True)
     for r in results.values():
         r["ctx"].close()
+
+
+def time_host_window(results, w, gt_u32, n, M, steps, log):
+    """Host window of each built precision: `steps` builds from pinned host
+    buffers (after one untimed), and one from pageable buffers; the link
+    rows are checked against the device entry's."""
+    import torch
+    from paragraph_amd import _native
+    dim = w["base"].shape[1]
+    base_p = _native.PinnedArray((n, dim), np.float32)
+    gt_p = _native.PinnedArray(gt_u32.shape, np.uint32)
+    base_p.a[:] = w["base"]
+    gt_p.a[:] = gt_u32
+    base_h = np.ascontiguousarray(w["base"])
+    out = {}
+    for prec, r in results.items():
+        ctx = r["ctx"]
+        proj_p = _native.PinnedArray((n, ctx.params.proj_deg), np.uint32)
+        link_p = _native.PinnedArray((n, M), np.uint32)
+        ctx.stage_host(base_p.a, gt_p.a, proj_p.a, link_p.a)
+        ctx.wait()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.stage_host(base_p.a, gt_p.a, proj_p.a, link_p.a)
+            ctx.wait()
+        ms = 1000.0 * (time.perf_counter() - t0) / steps
+        same = bool(np.array_equal(link_p.a,
+                                   r["link"].cpu().numpy().view(np.uint32)))
+        hproj = np.empty((n, ctx.params.proj_deg), np.uint32)
+        hlink = np.empty((n, M), np.uint32)
+        t1 = time.perf_counter()
+        ctx.stage_host(base_h, gt_u32, hproj, hlink)
+        ctx.wait()
+        ms_page = 1000.0 * (time.perf_counter() - t1)
+        same = same and bool(np.array_equal(hlink, link_p.a))
+        out[prec] = {"pinned_ms": round(ms, 2),
+                     "pinned_base_points_per_s": round(n / (ms / 1000.0), 1),
+                     "pageable_ms": round(ms_page, 2),
+                     "device_ms": round(r["ms_per_step"], 2),
+                     "same_rows_as_device_entry": same}
+        log(f"host window {prec}: {out[prec]}")
+        del proj_p, link_p
+    screened = [x for x in out if x != "fp32"]
+    if "fp32" in out and screened:
+        out[f"speedup_{screened[0]}_vs_fp32_pinned"] = round(
+            out["fp32"]["pinned_ms"] / out[screened[0]]["pinned_ms"], 3)
+    return out
 
 
 def beam_controls(results, p32, base_d, gt_d, n, M, L, dim, steps,

@@ -14,7 +14,9 @@ component choice for queries, ragged gt lengths padded with -1.
 Ground truth: exact top-k base ids by inner product (ties to the lower id),
 computed in float32 (on the GPU when one is present) and stored as int32 with
 -1 padding, the layout the reference reads (transform_gt_to_128.ipynb pads
-to 128 columns with -1).
+to 128 columns with -1): torch's float32 product for up to a few million
+rows, the library's exact top-k kernel (pg_exact_topk_ip, FP32 chain
+scores) for any size.
 """
 import numpy as np
 
@@ -124,6 +126,34 @@ def exact_topk_ip(queries, base, k, device=None, q_chunk=None):
     return out
 
 
+def exact_topk_ip_native(queries, base, k, device="cuda", q_chunk=1 << 20):
+    """Exact top-k ids by inner product with the library's GPU kernel
+    (pg_exact_topk_ip: BF16 MFMA screen with a rigorous bound, survivors
+    re-scored with the FP32 sequential chain, ties to the lower id), for
+    any N x N size; int32 with -1 padding."""
+    import torch
+    from . import _native
+    dev = torch.device(device)
+    B = torch.from_numpy(np.ascontiguousarray(base)).to(dev)
+    nq = queries.shape[0]
+    out = np.empty((nq, k), np.int32)
+    o = torch.empty((min(nq, q_chunk), k), dtype=torch.int32, device=dev)
+    for s in range(0, nq, q_chunk):
+        e = min(nq, s + q_chunk)
+        Q = torch.from_numpy(np.ascontiguousarray(queries[s:e])).to(dev)
+        exact_topk_ip_native.exact_rows += _native.exact_topk_ip(
+            Q, B, k, o[:e - s])
+        torch.cuda.synchronize(dev)
+        out[s:e] = o[:e - s].cpu().numpy()
+        del Q
+    del B, o
+    torch.cuda.empty_cache()
+    return out
+
+
+exact_topk_ip_native.exact_rows = 0  # rows the kernel's exact pass computed
+
+
 def subset_topk_ip(queries, base, k, subset, seed=19, device=None):
     """Sharded top-k by inner product (ties to the lower id): the base is
     split into random shards of about `subset` rows and query q takes the
@@ -160,10 +190,12 @@ def ragged_gt(gt, lo, hi, seed=17):
 
 def make_workload(n_base, dim, n_comp, n_query=None, gt_k=128,
                   normalize=False, zipf=None, gt_len_range=None,
-                  n_eval=0, device=None, with_gt=True, gt_subset=None):
+                  n_eval=0, device=None, with_gt=True, gt_subset=None,
+                  exact_native=False):
     """Returns dict(base, queries, gt (int32, -1 padded), eval_queries).
-    gt_subset: exact top-k within that many random base rows instead of all
-    (large builds)."""
+    gt_subset: exact top-k within that many random base rows instead of all;
+    exact_native: the exact gt from the library's GPU kernel (any size;
+    torch's float32 product otherwise)."""
     n_query = n_base if n_query is None else n_query
     base = base_vectors(n_base, dim, n_comp, normalize)
     queries = query_vectors(n_query, dim, n_comp, 15, normalize, zipf)
@@ -171,6 +203,9 @@ def make_workload(n_base, dim, n_comp, n_query=None, gt_k=128,
     if with_gt:
         if gt_subset is not None and gt_subset < n_base:
             gt = subset_topk_ip(queries, base, gt_k, gt_subset, device=device)
+        elif exact_native:
+            gt = exact_topk_ip_native(queries, base, gt_k,
+                                      device=device or "cuda")
         else:
             gt = exact_topk_ip(queries, base, gt_k, device)
         if gt_len_range is not None:
@@ -183,14 +218,15 @@ def make_workload(n_base, dim, n_comp, n_query=None, gt_k=128,
 
 
 def make_config(name, scale=1.0, n_eval=0, device=None, with_gt=True,
-                gt_subset=None):
+                gt_subset=None, exact_native=False):
     """One of CONFIGS, optionally scaled down in n_base (for tests)."""
     c = dict(CONFIGS[name])
     n = max(64, int(c["n_base"] * scale))
     w = make_workload(n, c["dim"], c["n_comp"], normalize=c.get("normalize",
                       False), zipf=c.get("zipf"),
                       gt_len_range=c.get("gt_len_range"), n_eval=n_eval,
-                      device=device, with_gt=with_gt, gt_subset=gt_subset)
+                      device=device, with_gt=with_gt, gt_subset=gt_subset,
+                      exact_native=exact_native)
     w["M"], w["L"] = c["M"], c["L"]
     w["cfg"] = c
     return w
