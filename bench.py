@@ -225,8 +225,10 @@ def main():
                          "the final index without link rows)")
     ap.add_argument("--gt-subset", type=int, default=None,
                     help="gt = exact top-128 within shards of this many base "
-                         "rows (default: exact over all rows up to 3M base "
-                         "rows, 200000-row shards above)")
+                         "rows (a quick stand-in; default: exact top-128 "
+                         "over all rows, by torch's FP32 product up to 3M "
+                         "base rows and by the library's exact top-k "
+                         "kernel above, about 8 minutes at 10M x 200)")
     ap.add_argument("--end-to-end", action="store_true",
                     help="also time the whole index build of the value "
                          "path, twice: with the host stages on the host "
@@ -249,11 +251,11 @@ def main():
     t0 = time.time()
     n_cfg = int(datagen.CONFIGS[args.config]["n_base"] * args.scale)
     gt_subset = args.gt_subset
-    if gt_subset is None and n_cfg > 3_000_000:
-        gt_subset = 200_000
     w = datagen.make_config(args.config, scale=args.scale,
                             n_eval=0 if args.no_recall else args.n_eval,
-                            device="cuda", gt_subset=gt_subset)
+                            device="cuda", gt_subset=gt_subset,
+                            exact_native=n_cfg > 3_000_000,
+                            log=lambda t: log(t))
     n, dim = w["base"].shape
     M, L = w["M"], w["L"]
     log(f"workload {args.config} n={n} dim={dim} M={M} L={L} generated in "
@@ -533,9 +535,12 @@ def main():
                        "prune)"}.get(head_prec, ""))
                    + (f"; screened beam L={head['beam']['L']}, re-rank "
                       f"width {head['beam']['topk']}"
-                      + (" (autotuned: fastest L whose final-index "
-                         "recall@10 is within min(0.3 pt, 5%) of the FP32 "
-                         "build's, on separate tuning queries)"
+                      + (" (autotuned: beams walked down from L in steps "
+                         "of 8, stopping at the first whose final-index "
+                         "recall@10 change against the FP32 build has a "
+                         "lower confidence bound (mean - 2 SE, paired, on "
+                         "separate tuning queries) below -min(0.5 pt, 5%); "
+                         "the fastest accepted beam)"
                          if tune_beam else "")
                       if head_prec != "fp32" else ""),
                    "n_base": n, "dim": dim, "M": M, "L": L,

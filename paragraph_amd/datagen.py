@@ -126,13 +126,18 @@ def exact_topk_ip(queries, base, k, device=None, q_chunk=None):
     return out
 
 
-def exact_topk_ip_native(queries, base, k, device="cuda", q_chunk=1 << 20):
+def exact_topk_ip_native(queries, base, k, device="cuda", q_chunk=1 << 20,
+                         log=None):
     """Exact top-k ids by inner product with the library's GPU kernel
     (pg_exact_topk_ip: BF16 MFMA screen with a rigorous bound, survivors
     re-scored with the FP32 sequential chain, ties to the lower id), for
-    any N x N size; int32 with -1 padding."""
+    any N x N size; int32 with -1 padding.  `log(text)` is called after
+    every chunk of q_chunk queries."""
+    import time
+
     import torch
     from . import _native
+    t0 = time.time()
     dev = torch.device(device)
     B = torch.from_numpy(np.ascontiguousarray(base)).to(dev)
     nq = queries.shape[0]
@@ -146,6 +151,9 @@ def exact_topk_ip_native(queries, base, k, device="cuda", q_chunk=1 << 20):
         torch.cuda.synchronize(dev)
         out[s:e] = o[:e - s].cpu().numpy()
         del Q
+        if log is not None:
+            log(f"exact gt: {e} of {nq} queries after {time.time() - t0:.0f}"
+                " s")
     del B, o
     torch.cuda.empty_cache()
     return out
@@ -191,11 +199,11 @@ def ragged_gt(gt, lo, hi, seed=17):
 def make_workload(n_base, dim, n_comp, n_query=None, gt_k=128,
                   normalize=False, zipf=None, gt_len_range=None,
                   n_eval=0, device=None, with_gt=True, gt_subset=None,
-                  exact_native=False):
+                  exact_native=False, log=None):
     """Returns dict(base, queries, gt (int32, -1 padded), eval_queries).
     gt_subset: exact top-k within that many random base rows instead of all;
     exact_native: the exact gt from the library's GPU kernel (any size;
-    torch's float32 product otherwise)."""
+    torch's float32 product otherwise); log: its progress callback."""
     n_query = n_base if n_query is None else n_query
     base = base_vectors(n_base, dim, n_comp, normalize)
     queries = query_vectors(n_query, dim, n_comp, 15, normalize, zipf)
@@ -205,7 +213,7 @@ def make_workload(n_base, dim, n_comp, n_query=None, gt_k=128,
             gt = subset_topk_ip(queries, base, gt_k, gt_subset, device=device)
         elif exact_native:
             gt = exact_topk_ip_native(queries, base, gt_k,
-                                      device=device or "cuda")
+                                      device=device or "cuda", log=log)
         else:
             gt = exact_topk_ip(queries, base, gt_k, device)
         if gt_len_range is not None:
@@ -218,7 +226,7 @@ def make_workload(n_base, dim, n_comp, n_query=None, gt_k=128,
 
 
 def make_config(name, scale=1.0, n_eval=0, device=None, with_gt=True,
-                gt_subset=None, exact_native=False):
+                gt_subset=None, exact_native=False, log=None):
     """One of CONFIGS, optionally scaled down in n_base (for tests)."""
     c = dict(CONFIGS[name])
     n = max(64, int(c["n_base"] * scale))
@@ -226,7 +234,7 @@ def make_config(name, scale=1.0, n_eval=0, device=None, with_gt=True,
                       False), zipf=c.get("zipf"),
                       gt_len_range=c.get("gt_len_range"), n_eval=n_eval,
                       device=device, with_gt=with_gt, gt_subset=gt_subset,
-                      exact_native=exact_native)
+                      exact_native=exact_native, log=log)
     w["M"], w["L"] = c["M"], c["L"]
     w["cfg"] = c
     return w
