@@ -237,7 +237,8 @@ def main():
                          "with them on the GPU (bipartite stage on a second "
                          "stream); reported in extra, never as value")
     ap.add_argument("--calib-copy", action="store_true",
-                    help="run a 1 GiB device copy first (counter calibration)")
+                    help="run a 1 GiB device copy and a 1 GiB sum first "
+                         "(counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "
@@ -264,12 +265,15 @@ def main():
     base_d = torch.from_numpy(w["base"]).to(dev)
     gt_d = torch.from_numpy(w["gt"]).to(dev)
     if args.calib_copy:
-        # a plain 1 GiB device copy, the reference point for HBM counters
+        # reference points for the HBM counters: a plain 1 GiB device copy
+        # (reads 1 GiB, writes 1 GiB) and a 1 GiB sum (reads 1 GiB only)
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
+        total = float(src.sum())
+        log(f"calibration: 1 GiB copy and 1 GiB sum ({total:.0f})")
         del src, dst
 
     precs = [x.strip() for x in args.precision.split(",") if x.strip()]

@@ -159,24 +159,38 @@ def test_whole_stage_matches_oracle(name, cuda):
         assert st[k] == so[k], (k, st[k], so[k])
 
 
-def test_cfg1_100k_whole_stage_matches_oracle(cuda):
-    """The 100K x 200 configuration at its stated size (64-component
-    mixture, rotated + offset training queries, exact top-128 inner-product
-    gt, M 32, L 100): the whole GPU stage, with the library's default
-    knobs and with the locality-ordered queue, equals the oracle's
-    projection and link rows id for id, counters included."""
+# (configuration, tune_flags besides the library's default)
+WHOLE_STAGE = [("cfg1_100k_d200", 0x624), ("cfg2_1m_d200", 0x224)]
+
+
+@pytest.mark.parametrize("cfg,flags2", WHOLE_STAGE,
+                         ids=[c for c, _ in WHOLE_STAGE])
+def test_config_whole_stage_matches_oracle(cfg, flags2, cuda):
+    """The 100K x 200 and 1M x 200 configurations at their stated sizes
+    (64-component mixture, rotated + offset training queries, exact
+    top-128 inner-product gt, M 32, L 100 / 128): the whole FP32 GPU
+    stage, with the library's default knobs and with the locality-ordered
+    queue (at 1M, the bench's tuned FP32 knobs), equals the oracle's
+    projection and link rows id for id over every node, counters
+    included."""
+    import time
+
     import torch
-    w = datagen.make_config("cfg1_100k_d200", device="cuda")
+    t0 = time.time()
+    w = datagen.make_config(cfg, device="cuda")
     base = w["base"]
     n, dim = base.shape
     gt = np.ascontiguousarray(w["gt"].view(np.uint32))
-    print(f"[parity] cfg1 whole stage n={n} dim={dim}", flush=True)
+    print(f"[parity] {cfg} whole stage n={n} dim={dim}: workload after "
+          f"{time.time() - t0:.0f} s", flush=True)
     p = _native.make_params(n, dim, n, w["M"], w["L"])
     st_o = cpu_ref.OrcStats()
     proj_o, link_o = cpu_ref.gpu_stage(orc_params(p), base, gt, stats=st_o)
+    print(f"[parity] {cfg}: oracle done after {time.time() - t0:.0f} s",
+          flush=True)
     base_d = to_dev(base, torch)
     gt_d = to_dev(gt, torch)
-    for flags in (p.tune_flags, 0x624):
+    for flags in (p.tune_flags, flags2):
         pp = _native.PgParams.from_buffer_copy(p)
         pp.tune_flags = flags
         proj_d = torch.empty((n, pp.proj_deg), dtype=torch.int32, device=cuda)
