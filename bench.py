@@ -414,8 +414,9 @@ def main():
                 f"{r['link_graph_recall_at_10']}")
             r["final_index_recall_at_10"] = final_index_recall(
                 final_index(proj_d, link_d), base_d, eq_d, truth, n, dim)
-            log(f"{prec}: recall@10 of the final index (reference "
-                "evaluation searcher: Km 128, Kp 6, Kd 64) = "
+            log(f"{prec}: recall@10 of the final index (the reference "
+                "evaluation searcher's knobs: Km 128, Kp 6, Kd 64, reset "
+                "every 4; visited table cleared per query) = "
                 f"{r['final_index_recall_at_10']}")
         if tuned is not None:
             r["autotune"] = tuned
@@ -490,6 +491,13 @@ def main():
         extra["end_to_end"] = end_to_end
     if controls is not None:
         extra["controls"] = controls
+    if truth is not None:
+        extra["final_index_searcher"] = (
+            "pg_run_search with search_shared_hashtable's knobs (Km 128, Kp "
+            "6, Kd 64, topk 128, reset every 4, entry ids (3030517 + i) mod "
+            "N) over the final lists padded to rows; each query starts from "
+            "an empty visited table, where the reference's warp keeps its "
+            "previous query's table until the first reset")
     for sp in screened:
         if "fp32" not in results:
             break

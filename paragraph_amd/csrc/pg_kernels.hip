@@ -1234,19 +1234,46 @@ __global__ __launch_bounds__(64) void merge_kernel(
     for (uint32_t j = lane; j < n2; j += kWave) buf[j] = j < seg ? srcs[s0 + j] : kTomb;
     wave_sync();
     if (seg > 1) sort_list<true>(buf, n2);
-  } else {
+  } else if (cap > 0) {
+    // A long segment (a hub: up to millions of sources).  buf[0, cap) holds
+    // the cap smallest ids seen so far, ascending, and thr its largest: a
+    // later id can only enter if it is below thr.  Such ids are staged
+    // after the list and merged in (one sort) when the next chunk could
+    // overflow the stage, so each chunk costs a compare per id.  Source ids
+    // are distinct (b -> c is one edge per pruned row), and with repeats
+    // the strict test still keeps the cap smallest of the multiset.
     for (uint32_t j = lane; j < kSegChunk; j += kWave) buf[j] = srcs[s0 + j];
     wave_sync();
     bitonic_sort<uint32_t, true>(buf, kSegChunk);
-    for (uint32_t done = kSegChunk; done < seg; done += kSegChunk) {
-      for (uint32_t j = lane; j < kSegChunk; j += kWave) {
-        if (j >= cap) buf[j] = kTomb;
-        const uint32_t g = done + j;
-        buf[kSegChunk + j] = g < seg ? srcs[s0 + g] : kTomb;
-      }
+    uint32_t thr = buf[cap - 1];
+    uint32_t ns = 0;  // staged ids at buf[cap, cap + ns)
+    auto fold = [&]() {
+      for (uint32_t j = cap + ns + lane; j < 2 * kSegChunk; j += kWave)
+        buf[j] = kTomb;
       wave_sync();
       bitonic_sort<uint32_t, true>(buf, 2 * kSegChunk);
+      thr = buf[cap - 1];
+      ns = 0;
+    };
+    for (uint32_t done = kSegChunk; done < seg; done += kSegChunk) {
+      uint32_t v[kSegChunk / kWave];
+#pragma unroll
+      for (uint32_t k = 0; k < kSegChunk / kWave; ++k) {
+        const uint32_t g = done + k * kWave + lane;
+        v[k] = g < seg ? srcs[s0 + g] : kTomb;
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < kSegChunk / kWave; ++k) {
+        const bool in = v[k] < thr;
+        const uint64_t im = __ballot(in);
+        if (in) buf[cap + ns + __popcll(im & lanemask_lt())] = v[k];
+        ns += __popcll(im);
+      }
+      if (cap + ns + kSegChunk > 2 * kSegChunk) fold();
     }
+    if (ns > 0) fold();
+    // buf[cap, kSegChunk) may hold staged or sorted ids past the list; the
+    // candidate list below reads only buf[0, r)
   }
   // ---- candidate list: reverse part then pruned part
   const uint32_t p = pcnt[c];

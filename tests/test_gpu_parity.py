@@ -505,3 +505,69 @@ def test_graph_replays_match_eager(prec, cuda):
             np.testing.assert_array_equal(from_dev(proj_g), ref[0])
             np.testing.assert_array_equal(from_dev(link_g), ref[1])
             _ = (base_d * 2.0).sum().item()  # eager work between replays
+
+
+@pytest.mark.parametrize("order", ["ascending", "descending", "shuffled",
+                                   "gpu_reverse"])
+@pytest.mark.parametrize("prec", [_native.PG_FP32, _native.PG_BF16])
+def test_merge_hub_segments_match_oracle(order, prec, cuda):
+    """Reverse segments past the 256-id chunk (four hubs with about 420 to
+    6 000 sources; the other nodes have a few): the merge keeps the cap smallest
+    sources whatever the order inside a segment (ascending: nothing
+    staged; descending: every id staged; shuffled; the GPU's own reverse
+    fill order), strict and non-strict, equal to the oracle's merge of the
+    ascending CSR over every node."""
+    import torch
+    rng = np.random.default_rng(21)
+    n, dim, mp = 6000, 16, 12
+    base = rng.standard_normal((n, dim)).astype(np.float32)
+    hubs = np.array([5, 77, 4001, 2999], np.uint32)
+    pr = np.full((n, mp), 0xFFFFFFFF, np.uint32)
+    pc = np.zeros(n, np.uint32)
+    for b in range(n):
+        row = []
+        for h, pin in zip(hubs, (1.0, 0.5, 0.14, 0.07)):
+            if h != b and rng.random() < pin:
+                row.append(int(h))
+        k = int(rng.integers(0, mp - len(row) + 1))
+        others = rng.choice(n, size=k + 4, replace=False)
+        for o in others:
+            if o != b and o not in row and len(row) < mp:
+                row.append(int(o))
+        pr[b, :len(row)] = row
+        pc[b] = len(row)
+    off, srcs = cpu_ref.reverse_csr(pr, pc)
+    seg = np.diff(off.astype(np.int64))
+    assert seg.max() > 20 * 256 and (seg[hubs] > 256).all()
+    if order in ("descending", "shuffled"):
+        s2 = srcs.copy()
+        for c in range(n):
+            a, e = int(off[c]), int(off[c + 1])
+            s2[a:e] = (s2[a:e][::-1] if order == "descending"
+                       else rng.permutation(s2[a:e]))
+        srcs_gpu = s2
+    else:
+        srcs_gpu = srcs
+    # the context's own M / L are unused by run_merge (cap, deg and the row
+    # width are its arguments)
+    p = _native.make_params(n, dim, n, 32, 32, precision=prec)
+    base_d = to_dev(base, torch)
+    pr_d, pc_d = to_dev(pr, torch), to_dev(pc, torch)
+    off_d = to_dev(off, torch)
+    srcs_d = to_dev(np.ascontiguousarray(srcs_gpu), torch)
+    with _native.Context(p) as ctx:
+        if order == "gpu_reverse":
+            srcs_d = torch.zeros(int(pc.sum()), dtype=torch.int32,
+                                 device=cuda)
+            ctx.run_reverse(pr_d, pc_d, off_d, srcs_d)
+            ctx.sync()
+            assert np.array_equal(from_dev(off_d), off)
+        for cap, deg, strict in ((100, 24, True), (64, 40, False),
+                                 (1, 8, True)):
+            out_d = torch.empty((n, deg), dtype=torch.int32, device=cuda)
+            ctx.run_merge(base_d, off_d, srcs_d, cap, pr_d, pc_d, deg, strict,
+                          out_d)
+            ctx.sync()
+            exp = cpu_ref.merge(base, off, srcs, cap, pr, pc, deg, strict)
+            np.testing.assert_array_equal(from_dev(out_d), exp,
+                                          err_msg=f"cap {cap}")
