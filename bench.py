@@ -192,7 +192,11 @@ def main():
                     help="comma list of builds to time (fp32, bf16, fp8); "
                          "value = the first screened one (bf16 / fp8), or "
                          "fp32 when it is the only one")
-    ap.add_argument("--cpu-sample", type=int, default=2000)
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="CPU baseline on this many random nodes, scaled "
+                         "to N (default: the whole workload in full up to "
+                         "1M base rows, about 30 s on 16 threads; a "
+                         "2000-node sample above)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recall", action="store_true")
     ap.add_argument("--no-dist-error", action="store_true")
@@ -456,13 +460,17 @@ def main():
                                      n, M, log)
         end_to_end.pop("_final", None)
 
-    # ---- CPU baseline: the oracle on a bounded node sample
+    # ---- CPU baseline: the oracle, in full up to 1M base rows, else on a
+    # bounded node sample
     cpu = None
     if not args.no_cpu_baseline:
-        ctx32 = results["fp32"]["ctx"] if "fp32" in results else \
-            _native.Context(p32)
-        cpu = cpu_baseline(ctx32, p32, w, gt_u32, base_d, gt_d,
-                           args.cpu_sample, log)
+        if args.cpu_sample is None and n <= 1_000_000:
+            cpu = cpu_full(p32, w, gt_u32, results.get("fp32"), log)
+        else:
+            ctx32 = results["fp32"]["ctx"] if "fp32" in results else \
+                _native.Context(p32)
+            cpu = cpu_baseline(ctx32, p32, w, gt_u32, base_d, gt_d,
+                               args.cpu_sample or 2000, log)
 
     screened = [x for x in precs if x != "fp32"]
     dist_err = None
@@ -829,6 +837,32 @@ def distance_error(ctxs, w, base_d, n_sample=2000):
                      "top10_overlap_in_gt128": round(float(ov), 4)}
     out["pairs"] = int(valid.sum())
     return out
+
+
+def cpu_full(p, w, gt_u32, r32, log):
+    """Oracle throughput on this host over the whole workload: the FP32
+    stage (match, projection, both link rounds) of every node, timed in
+    full; its rows are compared with the FP32 GPU build's (r32)."""
+    from oracle import cpu_ref
+    d = p.as_dict()
+    op = {k: d[k] for k, _ in cpu_ref.OrcParams._fields_}
+    n = w["base"].shape[0]
+    t0 = time.perf_counter()
+    proj, link = cpu_ref.gpu_stage(op, w["base"], gt_u32)
+    dt = time.perf_counter() - t0
+    same = None
+    if r32 is not None:
+        same = bool(np.array_equal(
+            proj, r32["proj"].cpu().numpy().view(np.uint32)) and
+            np.array_equal(link, r32["link"].cpu().numpy().view(np.uint32)))
+    log(f"cpu oracle, whole stage on all {n} nodes: {dt:.1f} s; rows equal "
+        f"to the FP32 GPU build's: {same}")
+    return {"value": round(n / dt, 1), "unit": "base points/s",
+            "cores": cpu_ref.max_threads(), "kind": "port",
+            "sample": f"the whole workload in full: oracle (cpu_ref, "
+                      f"C++/OpenMP) FP32 stage over all {n} nodes, "
+                      f"{dt:.1f} s",
+            "seconds": round(dt, 2), "rows_equal_fp32_gpu": same}
 
 
 def cpu_baseline(ctx, p, w, gt_u32, base_d, gt_d, sample, log):
