@@ -148,11 +148,14 @@ def roofline_of(ctx, p, n, workload, precision):
                       "% of MFMA cycles busy")
     if fr:
         bound = max(fr, key=lambda k: fr[k][0] / fr[k][1])
-    else:  # no profile for this kernel: the algorithmic byte rate only
-        bound, fr["hbm"] = "hbm", (alg_gbs, HBM_PEAK_GBS, "GB/s")
-    a, pk, unit = fr[bound]
+        a, pk, unit = fr[bound]
+        frac = round(a / pk, 4)
+    else:
+        # no profile of this kernel on this workload: only the algorithmic
+        # (cache-inclusive) byte rate, which names no bound
+        bound, a, pk, unit, frac = None, alg_gbs, HBM_PEAK_GBS, "GB/s", None
     roofline = {"bound": bound, "achieved": round(a, 2), "peak": pk,
-                "unit": unit, "frac": round(a / pk, 4),
+                "unit": unit, "frac": frac,
                 "traffic": int(traffic) if traffic else None,
                 "kernel": kname, "avg_launch_ms": round(avg_ms, 3),
                 "profile_avg_launch_ms": prof("profile_avg_launch_ms"),
