@@ -15,7 +15,14 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libparagraph_hip.so")
 ARCH = os.environ.get("PG_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pg_kernels.hip", "pg_topk.hip", "pg_capi.cpp"]
+# pg_search_*.hip hold the search kernels, one evaluator family each, so
+# that the largest instantiation sets compile in parallel
+HIP_SOURCES = ["pg_search_f32_16.hip", "pg_search_f32_8.hip",
+               "pg_search_f32_4.hip", "pg_search_bf16.hip",
+               "pg_search_fp8.hip", "pg_kernels.hip", "pg_search_f32_32.hip",
+               "pg_topk.hip", "pg_capi.cpp"]
+# parallel compiles (each hipcc holds a few GB at most)
+JOBS = max(1, min(8, os.cpu_count() or 1))
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -33,17 +40,23 @@ def _hipcc():
 
 
 def build(verbose=False):
+    from concurrent.futures import ThreadPoolExecutor
     hipcc = _hipcc()
-    objs = []
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    for src in HIP_SOURCES:
+
+    def compile_one(src):
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         cmd = [hipcc, "-x", "hip", f"--offload-arch={ARCH}", *COMMON,
                "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
+        return src, obj, subprocess.run(cmd, capture_output=True, text=True)
+
+    with ThreadPoolExecutor(max_workers=JOBS) as pool:
+        results = list(pool.map(compile_one, HIP_SOURCES))
+    objs = []
+    for src, obj, r in results:
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"hipcc failed on {src}")

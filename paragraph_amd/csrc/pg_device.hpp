@@ -291,4 +291,52 @@ __device__ __forceinline__ void stat_add(unsigned long long* stats, int field,
     atomicAdd(&stats[(blockIdx.x % kStatSlots) * kStatFields + field], v);
 }
 
+// Ascending sort of the n keys a[0..n) in LDS (n a power of two): in
+// registers (64*R keys, shuffle exchanges, no barriers between stages) for n
+// up to 512, else the LDS bitonic sort.  The caller's last LDS writes must be
+// behind a barrier.
+template <int R, bool kWaveSync = false, typename K>
+__device__ __forceinline__ void reg_sort_lds(K* a) {
+  const uint32_t lane = lane_id();
+  K v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = a[lane * R + r];
+  reg_bitonic_sort<R>(v);
+  lds_sync<kWaveSync>();
+#pragma unroll
+  for (int r = 0; r < R; ++r) a[lane * R + r] = v[r];
+  lds_sync<kWaveSync>();
+}
+
+template <bool kWaveSync = false, typename K>
+__device__ __forceinline__ void sort_list(K* a, uint32_t n) {
+  if (n == 64) reg_sort_lds<1, kWaveSync>(a);
+  else if (n == 128) reg_sort_lds<2, kWaveSync>(a);
+  else if (n == 256) reg_sort_lds<4, kWaveSync>(a);
+  else if (n == 512) reg_sort_lds<8, kWaveSync>(a);
+  else bitonic_sort<K, kWaveSync>(a, n);
+}
+
+// Screened RNG (the BF16 path): the same decisions as rng_select, with most
+// pair distances taken from BF16 MFMA Gram tiles.  Candidates are resolved
+
+// Elements e0 .. e0+7 of a query row (zero past dim or when !on): two
+// 16-byte loads when the row start is 16-byte aligned and dim % 8 == 0,
+// else scalar loads.
+__device__ __forceinline__ void load_query_octet(const float* __restrict__ x,
+                                                 uint32_t dim, bool on,
+                                                 uint32_t e0, float (&v)[8]) {
+  if (on && (dim & 7u) == 0 && e0 + 8u <= dim &&
+      (reinterpret_cast<uintptr_t>(x) & 15u) == 0) {
+    const float4 a = *reinterpret_cast<const float4*>(x + e0);
+    const float4 b = *reinterpret_cast<const float4*>(x + e0 + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      v[j] = (on && e0 + j < dim) ? x[e0 + j] : 0.0f;
+  }
+}
+
 }  // namespace pg
