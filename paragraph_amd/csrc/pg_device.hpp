@@ -97,6 +97,71 @@ __device__ __forceinline__ float chain_dist(const float* __restrict__ x,
   return -s;
 }
 
+// Lane k's value (k in 0..3) of the caller's quad, by a DPP quad broadcast.
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
+
+// chain_dist(x, y_q, dim) for one row y_q per quad (lanes 4q .. 4q+3), all
+// 64 lanes taking part (no lane may be inactive: the operands travel by DPP).
+// Lane k of a quad loads float4s 4j + k of its row, JB per batch, all in
+// flight together; then every lane of the quad runs the same sequential FMA
+// chain, taking float4 f from lane f % 4 by a quad broadcast and x from x_lds
+// (one row for the whole wave, in LDS: broadcast reads).  Same FMAs in the
+// same order as chain_dist, so the result is bit-equal.  Rows need dim % 4 ==
+// 0 (16-byte aligned float4s); other widths take chain_dist per lane.
+// Quads with !on return 0.
+template <int JB>
+__device__ __forceinline__ float quad_chain(const float* x_lds,
+                                            const float* __restrict__ y,
+                                            bool on, uint32_t dim) {
+  if (dim & 3u) return on ? chain_dist(x_lds, y, dim) : 0.0f;
+  const uint32_t k = threadIdx.x & 3u;
+  const uint32_t n4 = dim >> 2;
+  const float4* y4 = reinterpret_cast<const float4*>(y);
+  const float4* x4 = reinterpret_cast<const float4*>(x_lds);
+  float s = 0.0f;
+  for (uint32_t f0 = 0; f0 < n4; f0 += 4u * JB) {
+    float4 v[JB];
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      const uint32_t f = f0 + 4u * j + k;
+      v[j] = (on && f < n4) ? y4[f] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+        const uint32_t f = f0 + 4u * j + kk;
+        if (f < n4) {
+          const float4 a = x4[f];
+          float4 b;
+          if (kk == 0) {
+            b.x = quad_bcast<0>(v[j].x); b.y = quad_bcast<0>(v[j].y);
+            b.z = quad_bcast<0>(v[j].z); b.w = quad_bcast<0>(v[j].w);
+          } else if (kk == 1) {
+            b.x = quad_bcast<1>(v[j].x); b.y = quad_bcast<1>(v[j].y);
+            b.z = quad_bcast<1>(v[j].z); b.w = quad_bcast<1>(v[j].w);
+          } else if (kk == 2) {
+            b.x = quad_bcast<2>(v[j].x); b.y = quad_bcast<2>(v[j].y);
+            b.z = quad_bcast<2>(v[j].z); b.w = quad_bcast<2>(v[j].w);
+          } else {
+            b.x = quad_bcast<3>(v[j].x); b.y = quad_bcast<3>(v[j].y);
+            b.z = quad_bcast<3>(v[j].z); b.w = quad_bcast<3>(v[j].w);
+          }
+          s = __builtin_fmaf(a.x, b.x, s);
+          s = __builtin_fmaf(a.y, b.y, s);
+          s = __builtin_fmaf(a.z, b.z, s);
+          s = __builtin_fmaf(a.w, b.w, s);
+        }
+      }
+    }
+  }
+  return on ? -s : 0.0f;
+}
+
 // Value held by lane (lane ^ M), M in {1, 2, 4, 8, 16, 32}, without an LDS
 // round trip: DPP quad permutations for 1 and 2, DPP row rotate by 8 for 8
 // (within a 16-lane row, rotating by 8 is xor 8), ds_swizzle in bit-mask

@@ -566,7 +566,12 @@ __device__ __forceinline__ float wave_min_f32(float v) {
 // position << 9 | e).
 template <uint32_t DP2, int NCH, bool kScreen>
 struct SweepLds {
-  // B columns: 0 kept entry, 1 node, 2 / 3 their BF16 residuals
+  // B columns: 0 kept entry, 1 node, 2 / 3 their BF16 residuals, stored
+  // in the order (0, 2, 1, 3): slots 0-1 hold the kept entry's two columns,
+  // slots 2-3 the node's.  Once the first sweep has used the node's
+  // columns, slots 2-3 hold its FP32 row, and once a sweep has used the
+  // kept entry's, slots 0-1 may hold that entry's FP32 row (2 slots = dpad
+  // floats).
   bf16x8 frag[kScreen ? 4 * NCH * 4 : 1];
   float2 enr[kScreen ? DP2 : 1];  // (norm, residual norm)
   uint32_t eid[DP2];
@@ -575,7 +580,10 @@ struct SweepLds {
   uint32_t kept[128];
   uint16_t epos[DP2];
   uint16_t alive[DP2];
-  uint16_t rk[DP2];  // remover's kept index; 0x8000 | t for kept entry t
+  // remover's kept index; 0x8000 | t for kept entry t (the exact kernel's
+  // pair count)
+  uint16_t rk[kScreen ? 1 : DP2];
+  uint16_t wl[kScreen ? DP2 : 1];  // entries waiting for an exact chain
   uint8_t ef[DP2];   // bit 0: d exact; bit 1: kept or removed
 };
 
@@ -599,6 +607,7 @@ __device__ __forceinline__ uint32_t sweep_select(
   const float* xb = base + size_t(node) * dim;
   const uint32_t nch = kScreen ? sc.dpad / 32u : 0u;
   float nb = 0.0f, rb = 0.0f;
+  unsigned long long my_pairs = 0, my_fb = 0;
   // screened kernel, column col (0 kept entry / 1 node) from the FP32 row:
   // the BF16 column and its residual column (col + 2): hi = bf16(v) (the
   // conversion that made the BF16 copy), lo = bf16(v - hi)
@@ -614,17 +623,51 @@ __device__ __forceinline__ uint32_t sweep_select(
           h[j] = (__bf16)v[j];
           l[j] = (__bf16)(v[j] - (float)h[j]);
         }
-        L.frag[col * NF + lane] = h;
-        L.frag[(col + 2u) * NF + lane] = l;
+        L.frag[2u * col * NF + lane] = h;
+        L.frag[(2u * col + 1u) * NF + lane] = l;
       }
     }
   };
-  // exact d of entry e, and exact pair distance of e against the kept entry
+  // FP32 rows in LDS for the exact chains (screened kernel): the node's in
+  // slots 2-3, the current kept entry's in slots 0-1
+  float* const xnode = reinterpret_cast<float*>(L.frag + 2u * NF);
+  float* const xkept = reinterpret_cast<float*>(L.frag);
+  auto stage_row = [&](float* dst, uint32_t id) {
+    const float* x = base + size_t(id) * dim;
+    if ((dim & 3u) == 0) {
+      for (uint32_t f = lane; f < (dim >> 2); f += kWave)
+        reinterpret_cast<float4*>(dst)[f] =
+            reinterpret_cast<const float4*>(x)[f];
+    } else {
+      for (uint32_t i = lane; i < dim; i += kWave) dst[i] = x[i];
+    }
+  };
+  // Exact chains of the n entries listed in L.wl, 16 at a time (a quad per
+  // entry): d against the node row (slots 2-3; sets ef bit 0) or, with
+  // xs, the pair value against the kept row (slots 0-1) into L.sv.  Every
+  // lane calls it (quad_chain needs the whole wave).
+  auto wl_chains = [&](uint32_t n, bool xs) {
+    for (uint32_t b0 = 0; b0 < n; b0 += kWave / 4u) {
+      const uint32_t idx = b0 + (lane >> 2);
+      const bool on = idx < n;
+      const uint32_t e = on ? L.wl[idx] : 0u;
+      const float v = quad_chain<8>(xs ? xkept : xnode,
+                                    base + size_t(L.eid[e]) * dim, on, dim);
+      if (on && (lane & 3u) == 0) {
+        ++my_fb;
+        if (xs) {
+          L.sv[e] = v;
+        } else {
+          L.dv[e] = v;
+          L.ef[e] |= 1u;
+        }
+      }
+    }
+    wave_sync();
+  };
+  // exact d of entry e (one lane)
   auto exact_d = [&](uint32_t e) -> float {
     return chain_dist(xb, base + size_t(L.eid[e]) * dim, dim);
-  };
-  auto exact_s = [&](const float* xc, uint32_t e) -> float {
-    return chain_dist(xc, base + size_t(L.eid[e]) * dim, dim);
   };
   // bound of the screened d of entry e
   auto eb_of = [&](uint32_t e) -> float {
@@ -649,7 +692,6 @@ __device__ __forceinline__ uint32_t sweep_select(
       if (!(L.ef[e] & 1u)) return L.dv[e] + eb_of(e);
     return L.dv[e];
   };
-  unsigned long long my_pairs = 0, my_fb = 0;
   uint32_t nal = ne;
 
   // Pair values of every alive entry against entry c (the screened kernel
@@ -678,7 +720,9 @@ __device__ __forceinline__ uint32_t sweep_select(
 #pragma unroll
         for (int g = 0; g < G; ++g) {
           f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-          const bf16x8* bf = L.frag + (m < 4u ? m : 0u) * NF + kq;
+          // MFMA column m (< 4) is storage slot ((m & 1) << 1) | (m >> 1)
+          const uint32_t slot = m < 4u ? (((m & 1u) << 1) | (m >> 1)) : 0u;
+          const bf16x8* bf = L.frag + slot * NF + kq;
           __asm__ volatile("" ::: "memory");
 #pragma unroll
           for (int ch = 0; ch < NCH; ++ch)
@@ -747,6 +791,24 @@ __device__ __forceinline__ uint32_t sweep_select(
       nt += __popcll(tm);
     }
     if (nt == 1) return first_t;
+    if constexpr (kScreen) {
+      // exact d for the screened entries of that set, as quad chains
+      uint32_t nw = 0;
+      for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        uint32_t e = 0;
+        bool need = false;
+        if (i < nal) {
+          e = L.alive[i];
+          need = !(lo_of(e) > mh) && !(L.ef[e] & 1u);
+        }
+        const uint64_t bm = __ballot(need);
+        if (need) L.wl[nw + __popcll(bm & lanemask_lt())] = uint16_t(e);
+        nw += __popcll(bm);
+      }
+      wave_sync();
+      if (nw > 0) wl_chains(nw, false);
+    }
     uint64_t best = kPadKey;
     for (uint32_t i = lane; i < nal; i += kWave) {
       const uint32_t e = L.alive[i];
@@ -768,6 +830,11 @@ __device__ __forceinline__ uint32_t sweep_select(
   load_frag(0, L.eid[c]);
   wave_sync();
   sweep(c, true);
+  if constexpr (kScreen) {
+    // the node's columns are used: its FP32 row takes their slots
+    stage_row(xnode, node);
+    wave_sync();
+  }
   const uint32_t c1 = pick();
   if (c1 != c) {
     c = c1;
@@ -788,7 +855,7 @@ __device__ __forceinline__ uint32_t sweep_select(
     if ((L.ef[c] & 1u) && !(key_of(c) < kstop)) break;
     if (lane == 0) {
       L.kept[nk] = L.eid[c];
-      L.rk[c] = uint16_t(0x8000u | nk);
+      if constexpr (!kScreen) L.rk[c] = uint16_t(0x8000u | nk);
       L.ef[c] |= 2u;
     }
     ++nk;
@@ -796,46 +863,87 @@ __device__ __forceinline__ uint32_t sweep_select(
     wave_sync();
     if (nk == cap) break;
     // every alive entry against the new kept entry c
-    const float* xc = base + size_t(L.eid[c]) * dim;
     float nc = 0.0f, rc = 0.0f;
     if constexpr (kScreen) {
       nc = L.enr[c].x;
       rc = L.enr[c].y;
     }
-    for (uint32_t i = lane; i < nal; i += kWave) {
-      const uint32_t e = L.alive[i];
-      if (e == c) continue;
-      bool occl;
-      if constexpr (kScreen) {
-        ++my_pairs;
-        const float s = L.sv[e];
-        const float ec = pair_bound_res(sc, L.enr[e].x, L.enr[e].y, nc, rc);
+    if constexpr (kScreen) {
+      // Screened decisions (ef bit 1: removed).  An open interval needs the
+      // exact d first, then, if screen_cmp stays open, the exact pair value
+      // (ef bit 2 marks those); both run as quad chains over lists in L.wl.
+      auto decide = [&](uint32_t e, float s, float ec, bool& need_d) {
         if (s + ec < lo_of(e)) {
-          occl = true;
+          L.ef[e] |= 2u;
         } else if (s - ec >= hi_of(e)) {
-          occl = false;
+        } else if (!(L.ef[e] & 1u)) {
+          need_d = true;
         } else {
-          if (!(L.ef[e] & 1u)) {
-            L.dv[e] = exact_d(e);
-            L.ef[e] |= 1u;
-            ++my_fb;
-          }
-          int dec = screen_cmp(s, L.dv[e], ec);
-          if (dec < 0) {
-            ++my_fb;
-            dec = exact_s(xc, e) < L.dv[e];
-          }
-          occl = dec != 0;
+          const int dec = screen_cmp(s, L.dv[e], ec);
+          if (dec < 0) L.ef[e] |= 4u;
+          else if (dec) L.ef[e] |= 2u;
         }
-      } else {
-        occl = L.sv[e] < L.dv[e];
+      };
+      uint32_t nw = 0;
+      for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        bool need = false;
+        uint32_t e = 0;
+        if (i < nal) {
+          e = L.alive[i];
+          if (e != c) {
+            ++my_pairs;
+            decide(e, L.sv[e],
+                   pair_bound_res(sc, L.enr[e].x, L.enr[e].y, nc, rc), need);
+          }
+        }
+        const uint64_t bm = __ballot(need);
+        if (need) L.wl[nw + __popcll(bm & lanemask_lt())] = uint16_t(e);
+        nw += __popcll(bm);
       }
-      if (occl) {
-        L.ef[e] |= 2u;
-        L.rk[e] = uint16_t(nk - 1);
+      wave_sync();
+      if (nw > 0) {
+        wl_chains(nw, false);
+        for (uint32_t j = lane; j < nw; j += kWave) {
+          const uint32_t e = L.wl[j];
+          bool unused = false;
+          decide(e, L.sv[e],
+                 pair_bound_res(sc, L.enr[e].x, L.enr[e].y, nc, rc), unused);
+        }
+        wave_sync();
       }
+      uint32_t ns = 0;
+      for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
+        const uint32_t i = i0 + lane;
+        const uint32_t e = i < nal ? L.alive[i] : 0u;
+        const bool pend = i < nal && (L.ef[e] & 4u);
+        const uint64_t bm = __ballot(pend);
+        if (pend) L.wl[ns + __popcll(bm & lanemask_lt())] = uint16_t(e);
+        ns += __popcll(bm);
+      }
+      wave_sync();
+      if (ns > 0) {
+        // the kept entry's columns are used: its FP32 row takes their slots
+        stage_row(xkept, L.eid[c]);
+        wave_sync();
+        wl_chains(ns, true);
+        for (uint32_t j = lane; j < ns; j += kWave) {
+          const uint32_t e = L.wl[j];
+          L.ef[e] = uint8_t((L.ef[e] & ~4u) | (L.sv[e] < L.dv[e] ? 2u : 0u));
+        }
+      }
+      wave_sync();
+    } else {
+      for (uint32_t i = lane; i < nal; i += kWave) {
+        const uint32_t e = L.alive[i];
+        if (e == c) continue;
+        if (L.sv[e] < L.dv[e]) {
+          L.ef[e] |= 2u;
+          L.rk[e] = uint16_t(nk - 1);
+        }
+      }
+      wave_sync();
     }
-    wave_sync();
     uint32_t n2 = 0;
     for (uint32_t i0 = 0; i0 < nal; i0 += kWave) {
       const uint32_t i = i0 + lane;
