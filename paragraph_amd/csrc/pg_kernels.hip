@@ -845,12 +845,18 @@ __device__ __forceinline__ uint32_t sweep_select(
   while (true) {
     // stop rule: an entry whose key is not below kstop ends the scan
     if (kstop != kPadKey && !(L.ef[c] & 1u) && !(hi_of(c) < FLT_MAX)) {
-      if (lane == 0) {
-        L.dv[c] = exact_d(c);
-        L.ef[c] |= 1u;
-        ++my_fb;
+      if constexpr (kScreen) {
+        if (lane == 0) L.wl[0] = uint16_t(c);
+        wave_sync();
+        wl_chains(1, false);
+      } else {
+        if (lane == 0) {
+          L.dv[c] = exact_d(c);
+          L.ef[c] |= 1u;
+          ++my_fb;
+        }
+        wave_sync();
       }
-      wave_sync();
     }
     if ((L.ef[c] & 1u) && !(key_of(c) < kstop)) break;
     if (lane == 0) {
@@ -1275,7 +1281,8 @@ __global__ __launch_bounds__(64) void merge_kernel(
     int strict, uint32_t* __restrict__ out,
     unsigned long long* __restrict__ stats, ScreenArgs sc,
     const uint32_t* __restrict__ order) {
-  __shared__ uint32_t buf[2 * kSegChunk];
+  // 16-byte aligned: it also stages the node's FP32 row as float4s
+  __shared__ __align__(16) uint32_t buf[2 * kSegChunk];
   __shared__ uint64_t skey[128];
   __shared__ uint32_t sid[128];
   __shared__ uint32_t kept[128];
@@ -1339,15 +1346,41 @@ __global__ __launch_bounds__(64) void merge_kernel(
   // ---- candidate list: reverse part then pruned part
   const uint32_t p = pcnt[c];
   const uint32_t n = r + p;
-  const float* xc = base + size_t(c) * dim;
   for (uint32_t j = lane; j < 128; j += kWave) {
     uint32_t id = kTomb;
     if (j < r) id = buf[j];
     else if (j < n) id = pruned[size_t(c) * mp + (j - r)];
     sid[j] = id;
-    float d = FLT_MAX;
-    if (j < n) d = chain_dist(xc, base + size_t(id) * dim, dim);
-    skey[j] = j < n ? ((uint64_t(ordered_bits(d)) << 32) | j) : kPadKey;
+    if (j >= n) skey[j] = kPadKey;
+  }
+  wave_sync();
+  // exact d of every candidate: up to 32 candidates as quad chains (four
+  // lanes per candidate, its whole row in flight) against the node's row,
+  // staged in buf (dead once the sources are in sid); longer lists (where
+  // quads would take more passes than one lane per candidate takes load
+  // rounds) and wider rows take one lane per candidate
+  const float* xc = base + size_t(c) * dim;
+  if (n <= 32 && dim <= 2 * kSegChunk && (dim & 3u) == 0) {
+    float* xs = reinterpret_cast<float*>(buf);
+    for (uint32_t f = lane; f < (dim >> 2); f += kWave)
+      reinterpret_cast<float4*>(xs)[f] = reinterpret_cast<const float4*>(xc)[f];
+    wave_sync();
+    for (uint32_t b0 = 0; b0 < n; b0 += kWave / 4u) {
+      const uint32_t j = b0 + (lane >> 2);
+      const bool on = j < n;
+      const float d = quad_chain<8>(xs, base + size_t(on ? sid[j] : 0u) * dim,
+                                    on, dim);
+      if (on && (lane & 3u) == 0)
+        skey[j] = (uint64_t(ordered_bits(d)) << 32) | j;
+    }
+  } else {
+    // a fixed trip count (128 / 64 lanes) keeps both chains of a lane in
+    // flight together
+#pragma unroll
+    for (uint32_t j = lane; j < 128; j += kWave)
+      if (j < n)
+        skey[j] = (uint64_t(ordered_bits(chain_dist(
+                      xc, base + size_t(sid[j]) * dim, dim))) << 32) | j;
   }
   wave_sync();
   uint32_t* o = out + size_t(c) * deg;

@@ -407,13 +407,53 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
     }
   }
   bool unexp = false;
+  uint64_t k0 = kPadKey;  // top-list key of this lane (slice 0)
 #pragma unroll
   for (int s = 0; s < HPL; ++s) {
     const uint32_t i = lane + 64u * s;
-    if (i < Km && (keys[i] & 1ull) == 0) unexp = true;
+    if (i < Km) {
+      const uint64_t k = keys[i];
+      if (s == 0) k0 = k;
+      if ((k & 1ull) == 0) unexp = true;
+    }
   }
   if (ne == 0) return (force_final || __ballot(unexp) == 0) ? 1 : 0;
   if (force_final) return 1;
+  if constexpr (HPL == 1 && CH == 1) {
+    // Few entering keys (the common case at narrow beams): one uniform
+    // pass per entering key e (taken from its lane) finds its final
+    // position #(kept < e) + #(entering < e) from two ballots; lane p then
+    // takes the entering key placed at p, or else old key p - #(entering
+    // keys placed before p).  Same list as the general path below.
+    constexpr uint32_t kFewEntering = 8;
+    if (ne <= kFewEntering) {
+      uint64_t em = __ballot(in_e[0]);
+      uint32_t before = 0;
+      uint64_t sel = 0;
+      bool hit = false;
+      while (em) {
+        const int l = __ffsll((long long)em) - 1;
+        em &= em - 1;
+        const uint64_t e =
+            (uint64_t(__builtin_amdgcn_readlane(uint32_t(me[0] >> 32), l))
+             << 32) |
+            uint32_t(__builtin_amdgcn_readlane(uint32_t(me[0]), l));
+        const uint32_t pos = __popcll(__ballot(lane < Km && k0 < e)) +
+                             __popcll(__ballot(in_e[0] && me[0] < e));
+        before += pos < lane ? 1u : 0u;
+        if (pos == lane) {
+          sel = e;
+          hit = true;
+        }
+      }
+      // every lane reads its old key before any lane writes (the write
+      // depends on the read)
+      const uint64_t v = hit ? sel : (lane < Km ? keys[lane - before] : 0);
+      if (lane < Km) keys[lane] = v;
+      wave_sync();
+      return 0;
+    }
+  }
   wave_sync();  // ebuf overlays the todo list read above
 #pragma unroll
   for (int c = 0; c < CH; ++c)
