@@ -371,6 +371,9 @@ def main():
         from paragraph_amd import autotune
         if prec != "fp32" and args.beam is not None:
             p = autotune.beam_params(p, args.beam)
+            # explicit knobs win over the beam mapping's defaults
+            for k, v in ov.items():
+                setattr(p, k, int(v))
         if args.autotune:
             tuned = {}
             if prec != "fp32" and tune_beam:
