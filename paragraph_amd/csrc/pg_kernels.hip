@@ -707,9 +707,11 @@ __device__ __forceinline__ uint32_t sweep_select(
           const uint32_t id = ok ? L.eid[L.alive[idx]] : 0u;
           const bf16x8* row = reinterpret_cast<const bf16x8*>(
               sc.base16 + size_t(id) * sc.dpad + 8u * kq);
+          // lanes past nal read row 0 (a valid row): their output rows are
+          // never stored, so only the chunk bound (uniform) decides a load
 #pragma unroll
           for (int ch = 0; ch < NCH; ++ch) {
-            if (ok && uint32_t(ch) < nch) {
+            if (uint32_t(ch) < nch) {
               a[g][ch] = row[4 * ch];
             } else {
 #pragma unroll
@@ -1388,7 +1390,11 @@ __global__ __launch_bounds__(64) void merge_kernel(
     for (uint32_t j = lane; j < deg; j += kWave) o[j] = kTomb;
     return;
   }
-  sort_list<true>(skey, 128u);
+  // keys past n are padding, so the sort covers the next power of two
+  // >= n (at least 64: one register slice per lane)
+  uint32_t nk2 = 64;
+  while (nk2 < n) nk2 <<= 1;
+  sort_list<true>(skey, nk2);
   unsigned long long pairs = 0, fbk = 0;
   if (lane == 0) {
     kept[0] = sid[uint32_t(skey[0])];

@@ -173,9 +173,11 @@ struct EvalBF16 {
         const uint32_t cid = ok ? cand[todo[idx]] : 0u;
         const bf16x8* row = reinterpret_cast<const bf16x8*>(
             base16 + size_t(cid) * dpad + 8u * kq);
+        // lanes past ntodo read row 0 (a valid row): their output rows are
+        // never stored, so only the chunk bound (uniform) decides a load
 #pragma unroll
         for (int c = 0; c < NCH; ++c) {
-          if (ok && uint32_t(c) < nch) {
+          if (uint32_t(c) < nch) {
             a[g][c] = row[4 * c];
           } else {
 #pragma unroll
@@ -284,7 +286,8 @@ struct EvalFP8 {
             base8 + size_t(cid) * dpad + 8u * kq);
 #pragma unroll
         for (int c = 0; c < NCH; ++c)
-          a[g][c] = (ok && uint32_t(c) < nch) ? row[4 * c] : 0l;
+          // lanes past ntodo read row 0, whose output rows are never stored
+          a[g][c] = uint32_t(c) < nch ? row[4 * c] : 0l;
       }
 #pragma unroll
       for (int g = 0; g < G; ++g) {
