@@ -513,7 +513,13 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
 // orders its own LDS traffic with wave_sync, so the helpers, which wait at
 // the next barrier, never see its other phases.  Command 0 ends the helpers
 // (after wave 0's last query), so every wave passes the same barriers.
-template <class Eval, int HPL, int CH>
+//
+// kLean: a variant for the shapes the fast paths always cover (the host
+// picks it when Km <= 64*HPL, Kp*Kd <= 64*CH, the tail can stage Km keys
+// and max_iters > 0): only the rank merge and the register entry sort are
+// compiled, not the fallbacks, which leaves the compiler fewer live values
+// in the iteration loop.  Same lists.
+template <class Eval, int HPL, int CH, bool kLean = false>
 __global__ __launch_bounds__(128)
 __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     Eval ev, const float* __restrict__ queries, uint32_t n_query,
@@ -610,9 +616,9 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     }
   };
   while (q < n_query) {
-  const bool fast_merge = Km <= 64u * HPL && kpkd <= 64u * CH;
+  const bool fast_merge = kLean || (Km <= 64u * HPL && kpkd <= 64u * CH);
   // the rank merge stages the new top list in the tail region
-  const bool rank_merge = fast_merge && lp2 - Km >= Km;
+  const bool rank_merge = kLean || (fast_merge && lp2 - Km >= Km);
   ev.load_query(queries, q);
   // initial list: [0, Km) = (FLT_MAX, tomb); [Km, Km+kpkd) = entry points
   for (uint32_t i = lane; i < lp2; i += kWave)
@@ -629,7 +635,7 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
   // only the entry keys need sorting (in registers).  With max_iters == 0
   // the initial list is the output, which needs the full sort.
   constexpr int RS = CH <= 1 ? 1 : CH <= 2 ? 2 : 4;
-  if (max_iters > 0 && kpkd <= 64u * RS && Km <= 64u * RS) {
+  if (kLean || (max_iters > 0 && kpkd <= 64u * RS && Km <= 64u * RS)) {
     const uint64_t tomb_key = search_key(FLT_MAX, kTomb);
     uint64_t v[RS];
 #pragma unroll
@@ -644,7 +650,7 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
       if (e < Km) keys[e] = v[r] < tomb_key ? v[r] : tomb_key;
     }
     wave_sync();
-  } else {
+  } else if constexpr (!kLean) {
     bitonic_sort<uint64_t, true>(keys, lp2);
   }
   auto reset_table = [&]() {
@@ -750,13 +756,13 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     for (int c = 0; c < CH; ++c) {
       const uint32_t t = lane + 64u * c;
       const bool miss = vmiss[c];
-      if (t < kpkd && !miss) tail[t] = search_key(FLT_MAX, nid[c]);
+      // (a hit's FLT_MAX key is written only on the last iteration, below)
       const uint64_t m = __ballot(miss);
       if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = t;
       ntodo += __popcll(m);
     }
     // candidate slots past 64*CH (Kp*Kd up to 4096), one slice at a time
-    for (uint32_t c0 = 64u * CH; c0 < kpkd; c0 += kWave) {
+    for (uint32_t c0 = 64u * CH; !kLean && c0 < kpkd; c0 += kWave) {
       const uint32_t t = c0 + lane;
       bool miss = false;
       if (t < kpkd) {
@@ -778,10 +784,10 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     cache_count += ntodo;
     const bool last_allowed = total_iter + 1 >= max_iters;
     bool final_sort = true;
-    if (rank_merge) {
+    if (kLean || rank_merge) {
       final_sort = merge_ranked<CH, HPL>(keys, Km, tail, tail, todo, ntodo, ms,
                                          last_allowed) == 1;
-    } else if (fast_merge) {
+    } else if (!kLean && fast_merge) {
       if (ntodo <= 64)
         final_sort = merge_misses<1, HPL>(keys, Km, tail, todo, ntodo, ms,
                                         last_allowed);
@@ -792,7 +798,18 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
         final_sort = merge_misses<(CH <= 4 ? 4 : 8), HPL>(
             keys, Km, tail, todo, ntodo, ms, last_allowed);
     }
-    if (final_sort) sort_list<true>(keys, lp2);
+    if (final_sort) {
+      // The output's tail is the sorted union of the list and this last
+      // iteration's candidates, its visited hits at FLT_MAX: their keys
+      // are written here (misses hold their distances from the evaluator).
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const uint32_t t = lane + 64u * c;
+        if (t < kpkd && !vmiss[c]) tail[t] = search_key(FLT_MAX, nid[c]);
+      }
+      wave_sync();
+      sort_list<true>(keys, lp2);
+    }
     ++total_iter;
     if (Eval::kExactVisited && ++iter == reset_iter) {
       reset_table();
@@ -840,8 +857,11 @@ static hipError_t launch_search_eval(
   const int slices = (Km <= 64 && kpkd <= 64) ? 1
                      : (Km <= 128 && kpkd <= 128) ? 2
                      : (Km <= 192 && kpkd <= 192) ? 3 : 4;
+  // the lean one-slice variant where its fast paths always apply
+  const bool lean = slices == 1 && max_iters > 0 && lp2 - Km >= Km;
   const void* fn =
-      slices == 1 ? reinterpret_cast<const void*>(&search_kernel<Eval, 1, 1>)
+      lean ? reinterpret_cast<const void*>(&search_kernel<Eval, 1, 1, true>)
+      : slices == 1 ? reinterpret_cast<const void*>(&search_kernel<Eval, 1, 1>)
       : slices == 2 ? reinterpret_cast<const void*>(&search_kernel<Eval, 2, 2>)
       : slices == 3 ? reinterpret_cast<const void*>(&search_kernel<Eval, 3, 3>)
                     : reinterpret_cast<const void*>(&search_kernel<Eval, 4, 4>);
@@ -863,15 +883,16 @@ static hipError_t launch_search_eval(
     hipError_t e = launch_fill_u32(queue, kQueueSegs, 0u, s);
     if (e != hipSuccess) return e;
   }
-#define PG_SEARCH_LAUNCH(H, C)                                              \
-  search_kernel<Eval, H, C><<<grid, block, smem, s>>>(                      \
+#define PG_SEARCH_LAUNCH(H, C, LEAN)                                        \
+  search_kernel<Eval, H, C, LEAN><<<grid, block, smem, s>>>(                \
       ev, queries, n_query, n_base, graph, gdeg, Km, Kp, Kd, topk, reset_iter, \
       entry_seed, max_iters, table_size, lp2, kd_shift, out, stats, queue,  \
       queue ? order : nullptr)
-  if (slices == 1) PG_SEARCH_LAUNCH(1, 1);
-  else if (slices == 2) PG_SEARCH_LAUNCH(2, 2);
-  else if (slices == 3) PG_SEARCH_LAUNCH(3, 3);
-  else PG_SEARCH_LAUNCH(4, 4);
+  if (lean) PG_SEARCH_LAUNCH(1, 1, true);
+  else if (slices == 1) PG_SEARCH_LAUNCH(1, 1, false);
+  else if (slices == 2) PG_SEARCH_LAUNCH(2, 2, false);
+  else if (slices == 3) PG_SEARCH_LAUNCH(3, 3, false);
+  else PG_SEARCH_LAUNCH(4, 4, false);
 #undef PG_SEARCH_LAUNCH
   return hipGetLastError();
 }
