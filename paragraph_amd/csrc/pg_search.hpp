@@ -515,10 +515,11 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
 // (after wave 0's last query), so every wave passes the same barriers.
 //
 // kLean: a variant for the shapes the fast paths always cover (the host
-// picks it when Km <= 64*HPL, Kp*Kd <= 64*CH, the tail can stage Km keys
-// and max_iters > 0): only the rank merge and the register entry sort are
-// compiled, not the fallbacks, which leaves the compiler fewer live values
-// in the iteration loop.  Same lists.
+// picks it when Km <= 64*HPL, Kp*Kd <= 64*CH, the tail can stage Km keys,
+// max_iters > 0 and one wave per query): only the rank merge and the
+// register entry sort are compiled, not the fallbacks nor the evaluation
+// hand-off between waves, which leaves the compiler fewer live values in
+// the iteration loop.  Same lists.
 template <class Eval, int HPL, int CH, bool kLean = false>
 __global__ __launch_bounds__(128)
 __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
@@ -544,7 +545,9 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
   // sorted misses of the merge step: reuses cand + todo (2*kpkd words)
   uint64_t* ms = reinterpret_cast<uint64_t*>(cand);
   const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // the lean variant runs one wave per workgroup
+  const uint32_t wave = kLean ? 0u : threadIdx.x >> 6;
+  const uint32_t nw = kLean ? 1u : blockDim.x >> 6;
   uint64_t* tail = keys + Km;
   if (wave != 0) {
     uint32_t cur_q = 0xFFFFFFFFu;
@@ -858,7 +861,8 @@ static hipError_t launch_search_eval(
                      : (Km <= 128 && kpkd <= 128) ? 2
                      : (Km <= 192 && kpkd <= 192) ? 3 : 4;
   // the lean one-slice variant where its fast paths always apply
-  const bool lean = slices == 1 && max_iters > 0 && lp2 - Km >= Km;
+  const bool lean =
+      slices == 1 && max_iters > 0 && lp2 - Km >= Km && block == 64u;
   const void* fn =
       lean ? reinterpret_cast<const void*>(&search_kernel<Eval, 1, 1, true>)
       : slices == 1 ? reinterpret_cast<const void*>(&search_kernel<Eval, 1, 1>)
