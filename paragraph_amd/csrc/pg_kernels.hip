@@ -611,6 +611,10 @@ __device__ __forceinline__ uint32_t sweep_select(
   // screened kernel, column col (0 kept entry / 1 node) from the FP32 row:
   // the BF16 column and its residual column (col + 2): hi = bf16(v) (the
   // conversion that made the BF16 copy), lo = bf16(v - hi)
+  // The node's FP32 row, elements 8 lane .. +7 of lanes < NF (zero past
+  // dim), kept in registers from its fragment load until the first sweep
+  // has used the node's slots.
+  float nrow[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   auto load_frag = [&](uint32_t col, uint32_t id) {
     const float* x = base + size_t(id) * dim;
     if constexpr (kScreen) {
@@ -622,6 +626,7 @@ __device__ __forceinline__ uint32_t sweep_select(
         for (int j = 0; j < 8; ++j) {
           h[j] = (__bf16)v[j];
           l[j] = (__bf16)(v[j] - (float)h[j]);
+          if (col == 1u) nrow[j] = v[j];
         }
         L.frag[2u * col * NF + lane] = h;
         L.frag[(2u * col + 1u) * NF + lane] = l;
@@ -834,7 +839,12 @@ __device__ __forceinline__ uint32_t sweep_select(
   sweep(c, true);
   if constexpr (kScreen) {
     // the node's columns are used: its FP32 row takes their slots
-    stage_row(xnode, node);
+    if (lane < NF) {
+      reinterpret_cast<float4*>(xnode)[2u * lane] =
+          make_float4(nrow[0], nrow[1], nrow[2], nrow[3]);
+      reinterpret_cast<float4*>(xnode)[2u * lane + 1u] =
+          make_float4(nrow[4], nrow[5], nrow[6], nrow[7]);
+    }
     wave_sync();
   }
   const uint32_t c1 = pick();
