@@ -198,8 +198,11 @@ struct EvalBF16 {
         if (m < 4u) {
           const uint32_t cm = 4u * kq + m;
           const uint32_t idx = g0 + 16u * g + cm;
-          const float dot = m == 0 ? acc[0] : m == 1 ? acc[1]
-                          : m == 2 ? acc[2] : acc[3];
+          // element m of the accumulator, by selects (no branches)
+          float dot = acc[0];
+          dot = m == 1u ? acc[1] : dot;
+          dot = m == 2u ? acc[2] : dot;
+          dot = m == 3u ? acc[3] : dot;
           if (idx < ntodo) {
             const uint32_t t = todo[idx];
             tail[t] = search_key(-dot, cand[t]);
@@ -301,8 +304,11 @@ struct EvalFP8 {
         if (m < 4u) {
           const uint32_t cm = 4u * kq + m;
           const uint32_t idx = g0 + 16u * g + cm;
-          const float dot = m == 0 ? acc[0] : m == 1 ? acc[1]
-                          : m == 2 ? acc[2] : acc[3];
+          // element m of the accumulator, by selects (no branches)
+          float dot = acc[0];
+          dot = m == 1u ? acc[1] : dot;
+          dot = m == 2u ? acc[2] : dot;
+          dot = m == 3u ? acc[3] : dot;
           if (idx < ntodo) {
             const uint32_t t = todo[idx];
             const uint32_t cid = cand[t];
@@ -657,7 +663,10 @@ __attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
     bitonic_sort<uint64_t, true>(keys, lp2);
   }
   auto reset_table = [&]() {
-    for (uint32_t i = lane; i < tsize; i += kWave) tab[i] = 0u;
+    // 16-byte stores (the table starts 16-byte aligned), then the rest
+    for (uint32_t i = lane; i < (tsize >> 2); i += kWave)
+      reinterpret_cast<uint4*>(tab)[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t i = (tsize & ~3u) + lane; i < tsize; i += kWave) tab[i] = 0u;
     wave_sync();
     for (uint32_t i = lane; i < Km; i += kWave)
       visit_insert(tab, tsize, key_raw_id(keys[i]) & kMaskedTomb);
