@@ -48,6 +48,7 @@ template <int TQ, int U>
 struct EvalF32 {
   static constexpr bool kExactVisited = true;
   static constexpr int kMinWaves = 1;  // occupancy target for the compiler
+  static constexpr int kLeanMinWaves = 1;  // the same for the lean variant
   static constexpr uint32_t kQueryLdsBytes = 0;
   bf16x8* qs;  // unused
   float qv[TQ];
@@ -130,6 +131,8 @@ struct EvalBF16 {
   // padded elements: 4 waves (<= 128 VGPRs, no spills; 3 untargeted)
   static constexpr int kMinWaves =
       (NCH <= 8 && G == 1) ? 6 : (NCH <= 16 && G == 1) ? 4 : 1;
+  // the lean variant holds fewer live values: 7 waves (<= 72 VGPRs)
+  static constexpr int kLeanMinWaves = (NCH <= 8 && G == 1) ? 7 : kMinWaves;
   // The query's B-operand fragments live in LDS (qs[4c + k] = elements
   // 32c + 8k .. +7), read per MFMA: replicating them in registers would
   // cost 4*NCH VGPRs per lane.
@@ -231,6 +234,7 @@ template <int NCH, int G>
 struct EvalFP8 {
   static constexpr bool kExactVisited = false;
   static constexpr int kMinWaves = (NCH <= 8 && G == 1) ? 5 : 1;
+  static constexpr int kLeanMinWaves = kMinWaves;
   // e4m3 query fragments in LDS: 8 bytes per (chunk, lane group)
   static constexpr uint32_t kQueryLdsBytes = NCH * 4 * 8;
   bf16x8* qs;
@@ -528,7 +532,8 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
 // the iteration loop.  Same lists.
 template <class Eval, int HPL, int CH, bool kLean = false>
 __global__ __launch_bounds__(128)
-__attribute__((amdgpu_waves_per_eu(Eval::kMinWaves, 8))) void search_kernel(
+__attribute__((amdgpu_waves_per_eu(kLean ? Eval::kLeanMinWaves
+                                         : Eval::kMinWaves, 8))) void search_kernel(
     Eval ev, const float* __restrict__ queries, uint32_t n_query,
     uint32_t n_base,
     const uint32_t* __restrict__ graph, uint32_t gdeg, uint32_t Km,
