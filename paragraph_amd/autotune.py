@@ -147,13 +147,19 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     # prune variants on the search rows of every base row (the prune's cost
     # depends on cache reuse across the whole node range, so a sample with
     # empty rows elsewhere misranks them): kept-major sweep or
-    # candidate-major scan, and for screened builds the split screen
+    # candidate-major scan, and for screened builds the split screen.  The
+    # split screen also decides the merge's screen (the sweep screens
+    # plainly either way), so each variant is timed on the link round's
+    # prune, reverse lists and merge together.
     full = torch.empty((n, p.topk), **i32)
     with _native.Context(p) as ctx:
         ctx.run_search(base_d, base_d, link, full)
         ctx.sync()
     pr = torch.empty((n, p.M), **i32)
     pc = torch.empty(n, **i32)
+    roff = torch.empty(n + 1, **i32)
+    rsrcs = torch.empty(n * p.M, **i32)
+    mout = torch.empty((n, p.M), **i32)
     splits = (0, 0x80) if p.precision != _native.PG_FP32 else (0,)
     best_pf, best_pt = 0, float("inf")
     grid = best["tune_flags"] & 0x220 if best is not None else 0x20
@@ -164,13 +170,18 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             # takes its nodes in that order
             pp.tune_flags = ((p.tune_flags & ~0x3A0) | grid | split | scan)
             with _native.Context(pp) as ctx:
-                t = _time(lambda: (ctx.run_prune_rows(base_d, full, p.M, pr,
-                                                      pc), ctx.sync()))
+                t = _time(lambda: (
+                    ctx.run_prune_rows(base_d, full, p.M, pr, pc),
+                    ctx.run_reverse(pr, pc, roff, rsrcs),
+                    ctx.run_merge(base_d, roff, rsrcs, p.link_rev, pr, pc,
+                                  p.M, True, mout),
+                    ctx.sync()))
             report["prune"].append({"split": bool(split), "scan": bool(scan),
-                                    "ms": round(1000 * t, 3)})
+                                    "ms_prune_reverse_merge":
+                                    round(1000 * t, 3)})
             if log:
-                log(f"autotune prune split={bool(split)} scan={bool(scan)}: "
-                    f"{1000 * t:.2f} ms")
+                log(f"autotune prune + reverse + merge split={bool(split)} "
+                    f"scan={bool(scan)}: {1000 * t:.2f} ms")
             if t < best_pt:
                 best_pf, best_pt = split | scan, t
     best["tune_flags"] |= best_pf

@@ -241,11 +241,20 @@ pg::ScreenArgs screen_args(const pg_ctx* c) {
                         c->gam_a, c->gam_c};
 }
 
+// The link-row sweep always screens with the plain BF16 copy and the
+// residual-norm bound (its exact chains run as quad chains): a split-screen
+// context keeps the split for the merge and the candidate-major prune.
+pg::ScreenArgs sweep_screen_args(const pg_ctx* c) {
+  pg::ScreenArgs a = screen_args(c);
+  a.base16lo = nullptr;
+  a.rnorms = c->rnorms;
+  return a;
+}
+
 int prepare_screen(pg_ctx* c, const float* d_base) {
   PG_HIP(c, pg::launch_prepare_screen(d_base, c->p.n_base, c->p.dim, c->dpad,
                                       c->base16, c->base16lo, c->norms,
-                                      c->base16lo ? nullptr : c->rnorms,
-                                      c->stream));
+                                      c->rnorms, c->stream));
   if (c->base8)
     PG_HIP(c, pg::launch_prepare_fp8(d_base, c->p.n_base, c->p.dim, c->dpad,
                                      c->base8, c->scale8, c->stream));
@@ -289,6 +298,11 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
   PG_TRY(record(c, ev++));
   const pg::ScreenArgs sc = screen_args(c);
   const pg::ScreenArgs* scp = c->base16 ? &sc : nullptr;
+  // the link-row prune: the sweep screens plainly, the candidate-major
+  // kernel with the context's screen
+  const bool sweep = !(p.tune_flags & kTuneScanPrune);
+  const pg::ScreenArgs scw = sweep_screen_args(c);
+  const pg::ScreenArgs* scl = c->base16 ? (sweep ? &scw : &sc) : nullptr;
   if (!prologue_done) {
     if (c->base16)  // screened path: BF16 split of the base and row norms
       PG_TRY(prepare_screen(c, d_base));
@@ -330,8 +344,7 @@ int run_pipeline(pg_ctx* c, const float* d_base, const uint32_t* d_gt,
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_prune(d_base, N, p.dim, c->res, width, nullptr,
                                nullptr, 0, p.M, c->pruned, c->pcnt, c->stats,
-                               scp, !(p.tune_flags & kTuneScanPrune), sorder,
-                               s));
+                               scl, sweep, sorder, s));
     PG_TRY(record(c, ev++));
     PG_HIP(c, pg::launch_reverse(c->pruned, c->pcnt, N, p.M, c->deg, c->off,
                                  c->cursor, c->scan_tmp, c->srcs, s));
@@ -737,8 +750,7 @@ int pg_gpu_stage_async(pg_ctx* c, const float* base, const uint32_t* gt,
                       static_cast<uint16_t*>(c->base16) + r0 * d,
                       c->base16lo ? static_cast<uint16_t*>(c->base16lo) + r0 * d
                                   : nullptr,
-                      c->norms + r0,
-                      c->base16lo ? nullptr : c->rnorms + r0, s));
+                      c->norms + r0, c->rnorms + r0, s));
         if (c->base8)
           PG_HIP(c, pg::launch_prepare_fp8(
                         c->h_base_dev + r0 * p.dim, uint32_t(nr), p.dim,
@@ -879,15 +891,18 @@ int pg_run_prune(pg_ctx* c, const float* d_base, const uint32_t* d_rows,
   if (D == 0 || D > 512) return set_err(c, PG_ERR_INVALID, "D must be in [1, 512]");
   if (mp < 2 || mp > 128) return set_err(c, PG_ERR_INVALID, "mp must be in [2, 128]");
   PG_HIP(c, hipSetDevice(c->device));
-  const pg::ScreenArgs sc = screen_args(c);
+  const bool sweep = !(c->p.tune_flags & kTuneScanPrune);
+  // the sweep screens plainly (link rows), the candidate-major kernel with
+  // the context's screen
+  const pg::ScreenArgs sc =
+      (sweep && d_rows) ? sweep_screen_args(c) : screen_args(c);
   if (c->base16) PG_TRY(prepare_screen(c, d_base));
   // link rows take nodes in the locality order when the context has one
   const uint32_t* sorder = nullptr;
   if (d_rows) PG_TRY(search_order(c, d_base, c->p.n_base, &sorder, true));
   PG_HIP(c, pg::launch_prune(d_base, c->p.n_base, c->p.dim, d_rows, D, d_gt,
                              d_winner, gt_k, mp, d_pruned, d_pcnt, c->stats,
-                             c->base16 ? &sc : nullptr,
-                             !(c->p.tune_flags & kTuneScanPrune), sorder,
+                             c->base16 ? &sc : nullptr, sweep, sorder,
                              c->stream));
   return PG_OK;
 }

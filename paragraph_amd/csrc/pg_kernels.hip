@@ -1513,8 +1513,8 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
   if (D > 512) return hipErrorInvalidValue;
   if (rows != nullptr && sweep) {
     // kept-major sweep: screened on the plain BF16 bound; exact FP32 chains
-    // for the FP32 path and for split-screen contexts (whose hi + lo rows
-    // cost as many bytes as the FP32 row)
+    // for the FP32 path and when handed split-screen arguments (the C ABI
+    // hands the sweep plain-screen arguments in every screened context)
 #define PG_SWEEP(DP, NC, SCR)                                        \
   prune_sweep_kernel<DP, NC, SCR><<<n_base, 64, 0, s>>>(              \
       base, n_base, dim, rows, D, mp, pruned, pcnt, stats, scv, order)
