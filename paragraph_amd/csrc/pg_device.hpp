@@ -327,6 +327,36 @@ __device__ __forceinline__ void reg_bitonic_sort(K (&v)[R]) {
   }
 }
 
+// Ascending position of this lane's value among the values of lanes
+// [0, n) (n <= 64, uniform): #(lanes i < n whose (value, i) is below this
+// lane's (value, lane)), so equal values take consecutive positions in lane
+// order and the n positions are a permutation of [0, n).  One readlane and
+// a compare per value: for short lists this is far cheaper than a 64-wide
+// bitonic network.  Every lane must take part; lanes >= n get an unused
+// result.
+__device__ __forceinline__ uint32_t wave_rank(uint32_t v, uint32_t n) {
+  const uint32_t lane = lane_id();
+  uint32_t r = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t a = __builtin_amdgcn_readlane(v, i);
+    r += (a < v || (a == v && i < lane)) ? 1u : 0u;
+  }
+  return r;
+}
+// The same for 64-bit keys that are distinct among lanes [0, n).
+__device__ __forceinline__ uint32_t wave_rank_distinct(uint64_t v,
+                                                       uint32_t n) {
+  const uint32_t hi = uint32_t(v >> 32), lo = uint32_t(v);
+  uint32_t r = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t a =
+        (uint64_t(__builtin_amdgcn_readlane(hi, i)) << 32) |
+        uint32_t(__builtin_amdgcn_readlane(lo, i));
+    r += a < v ? 1u : 0u;
+  }
+  return r;
+}
+
 // Number of entries of the sorted LDS array a[0..n) that are < x
 // (lower bound) or <= x (upper bound).
 __device__ __forceinline__ uint32_t count_less(const uint64_t* a, uint32_t n,

@@ -1283,6 +1283,10 @@ __global__ void scan_add_kernel(uint32_t* __restrict__ out, uint32_t n,
 // waves per SIMD instead of three (BF16 merge 4.8 / 5.1 -> 4.2 / 4.5 ms per
 // round on 1M x 200).
 constexpr uint32_t kSegChunk = 256;
+// Lists of at most this many sources / candidates are ordered by rank
+// placement (wave_rank) instead of a 64-wide bitonic sort: on the 1M x 200
+// value path the merge's lists average under 4 candidates.
+constexpr uint32_t kRankSortMax = 32;
 
 template <int NCH, int NKT, bool kSplit>
 __global__ __launch_bounds__(64) void merge_kernel(
@@ -1308,7 +1312,13 @@ __global__ __launch_bounds__(64) void merge_kernel(
   const uint32_t s0 = off[c], seg = off[c + 1] - s0;
   const uint32_t r = seg < cap ? seg : cap;
   // ---- select the r smallest sources, ascending
-  if (seg <= kSegChunk) {
+  if (seg <= kRankSortMax) {
+    // a short segment (most nodes): each source placed at its rank
+    const uint32_t v = lane < seg ? srcs[s0 + lane] : kTomb;
+    const uint32_t rk = wave_rank(v, seg);
+    if (lane < seg) buf[rk] = v;
+    wave_sync();
+  } else if (seg <= kSegChunk) {
     uint32_t n2 = 64;
     while (n2 < seg) n2 <<= 1;
     for (uint32_t j = lane; j < n2; j += kWave) buf[j] = j < seg ? srcs[s0 + j] : kTomb;
@@ -1400,11 +1410,21 @@ __global__ __launch_bounds__(64) void merge_kernel(
     for (uint32_t j = lane; j < deg; j += kWave) o[j] = kTomb;
     return;
   }
-  // keys past n are padding, so the sort covers the next power of two
-  // >= n (at least 64: one register slice per lane)
-  uint32_t nk2 = 64;
-  while (nk2 < n) nk2 <<= 1;
-  sort_list<true>(skey, nk2);
+  if (n <= kRankSortMax) {
+    // a short list (most nodes): each key placed at its rank (keys are
+    // distinct: the low word is the candidate's position)
+    const uint64_t k = lane < n ? skey[lane] : kPadKey;
+    const uint32_t rk = wave_rank_distinct(k, n);
+    wave_sync();
+    if (lane < n) skey[rk] = k;
+    wave_sync();
+  } else {
+    // keys past n are padding, so the sort covers the next power of two
+    // >= n (at least 64: one register slice per lane)
+    uint32_t nk2 = 64;
+    while (nk2 < n) nk2 <<= 1;
+    sort_list<true>(skey, nk2);
+  }
   unsigned long long pairs = 0, fbk = 0;
   if (lane == 0) {
     kept[0] = sid[uint32_t(skey[0])];
