@@ -1928,9 +1928,9 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(
     Eval ev, const float* __restrict__ queries, uint32_t n_query,
     const uint32_t* __restrict__ ids, uint32_t k, uint32_t n_base,
     float* __restrict__ out) {
-  __shared__ uint32_t cand[kPairMaxK];
-  __shared__ uint32_t todo[kPairMaxK];
-  __shared__ uint64_t tail[kPairMaxK];
+  __shared__ uint32_t todo[kPairMaxK];  // ids of the valid pairs
+  __shared__ uint32_t slot[kPairMaxK];  // their positions in the row
+  __shared__ uint64_t tail[kPairMaxK];  // their keys, in todo order
   __shared__ uint32_t ntodo_s;
   // query fragments of evaluators that keep them in LDS
   __shared__ bf16x8 qbuf[Eval::kQueryLdsBytes ? Eval::kQueryLdsBytes / 16 : 1];
@@ -1943,26 +1943,25 @@ __global__ __launch_bounds__(64) void pair_dist_kernel(
     const uint32_t t = t0 + lane;
     const uint32_t id = t < k ? ids[size_t(q) * k + t] : kTomb;
     const bool ok = t < k && id < n_base;
-    if (t < k) {
-      cand[t] = ok ? id : 0u;
-      tail[t] = kPadKey;
-    }
+    if (t < k && !ok) out[size_t(q) * k + t] = __builtin_nanf("");
     const uint64_t bal = __ballot(ok);
     const uint32_t base_pos = ntodo_s;
     __syncthreads();
-    if (ok) todo[base_pos + __popcll(bal & lanemask_lt())] = t;
+    if (ok) {
+      const uint32_t i = base_pos + __popcll(bal & lanemask_lt());
+      todo[i] = id;
+      slot[i] = t;
+    }
     if (lane == 0) ntodo_s = base_pos + __popcll(bal);
     __syncthreads();
   }
   const uint32_t ntodo = ntodo_s;
   ev.load_query(queries, q);
   __syncthreads();
-  ev.eval(cand, todo, ntodo, tail);
+  ev.eval(todo, ntodo, tail);
   __syncthreads();
-  for (uint32_t t = lane; t < k; t += kWave) {
-    const uint64_t key = tail[t];
-    out[size_t(q) * k + t] = key == kPadKey ? __builtin_nanf("") : key_dist(key);
-  }
+  for (uint32_t i = lane; i < ntodo; i += kWave)
+    out[size_t(q) * k + slot[i]] = key_dist(tail[i]);
 }
 
 // FP32 chain distance of the prune / merge stages for the same pairs.

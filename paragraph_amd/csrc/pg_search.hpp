@@ -70,24 +70,23 @@ struct EvalF32 {
     }
   }
 
-  // Candidates todo[0, ntodo) split over `parts` waves: wave `part` takes
-  // the candidate pairs p0 = 2U * (part + parts * i).
-  __device__ __forceinline__ void eval(const uint32_t* cand,
-                                       const uint32_t* todo, uint32_t ntodo,
+  // Candidate ids todo[0, ntodo) split over `parts` waves: wave `part`
+  // takes the candidate pairs p0 = 2U * (part + parts * i); the key of
+  // todo[i] goes to tail[i].
+  __device__ __forceinline__ void eval(const uint32_t* todo, uint32_t ntodo,
                                        uint64_t* tail, uint32_t part = 0,
                                        uint32_t parts = 1) const {
     const uint32_t lane = lane_id();
     const uint32_t half = lane >> 5, l = lane & 31u;
     for (uint32_t p0 = 2 * U * part; p0 < ntodo; p0 += 2 * U * parts) {
-      uint32_t tt[U], cid[U];
+      uint32_t cid[U];
       bool ok[U];
       float xv[U][TQ];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const uint32_t idx = p0 + 2 * u + half;
         ok[u] = idx < ntodo;
-        tt[u] = ok[u] ? todo[idx] : 0u;
-        cid[u] = ok[u] ? cand[tt[u]] : 0u;
+        cid[u] = ok[u] ? todo[idx] : 0u;
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -110,7 +109,7 @@ struct EvalF32 {
         s = s + xor_lane<4>(s);
         s = s + xor_lane<2>(s);
         s = s + xor_lane<1>(s);
-        if (l == 0 && ok[u]) tail[tt[u]] = search_key(-s, cid[u]);
+        if (l == 0 && ok[u]) tail[p0 + 2 * u + half] = search_key(-s, cid[u]);
       }
     }
   }
@@ -159,10 +158,10 @@ struct EvalBF16 {
   __device__ __forceinline__ void load_query_regs(const float* __restrict__,
                                                   uint32_t) {}
 
-  // Groups of 16 candidates split over `parts` waves: wave `part` takes the
-  // groups g0 = 16G * (part + parts * i).
-  __device__ __forceinline__ void eval(const uint32_t* cand,
-                                       const uint32_t* todo, uint32_t ntodo,
+  // Groups of 16 candidate ids split over `parts` waves: wave `part` takes
+  // the groups g0 = 16G * (part + parts * i); the key of todo[i] goes to
+  // tail[i].
+  __device__ __forceinline__ void eval(const uint32_t* todo, uint32_t ntodo,
                                        uint64_t* tail, uint32_t part = 0,
                                        uint32_t parts = 1) const {
     const uint32_t lane = lane_id();
@@ -173,7 +172,7 @@ struct EvalBF16 {
       for (int g = 0; g < G; ++g) {
         const uint32_t idx = g0 + 16u * g + m;
         const bool ok = idx < ntodo;
-        const uint32_t cid = ok ? cand[todo[idx]] : 0u;
+        const uint32_t cid = ok ? todo[idx] : 0u;
         const bf16x8* row = reinterpret_cast<const bf16x8*>(
             base16 + size_t(cid) * dpad + 8u * kq);
         // lanes past ntodo read row 0 (a valid row): their output rows are
@@ -206,10 +205,7 @@ struct EvalBF16 {
           dot = m == 1u ? acc[1] : dot;
           dot = m == 2u ? acc[2] : dot;
           dot = m == 3u ? acc[3] : dot;
-          if (idx < ntodo) {
-            const uint32_t t = todo[idx];
-            tail[t] = search_key(-dot, cand[t]);
-          }
+          if (idx < ntodo) tail[idx] = search_key(-dot, todo[idx]);
         }
       }
     }
@@ -276,8 +272,8 @@ struct EvalFP8 {
     }
   }
 
-  __device__ __forceinline__ void eval(const uint32_t* cand,
-                                       const uint32_t* todo, uint32_t ntodo,
+  // Same split and outputs as EvalBF16::eval.
+  __device__ __forceinline__ void eval(const uint32_t* todo, uint32_t ntodo,
                                        uint64_t* tail, uint32_t part = 0,
                                        uint32_t parts = 1) const {
     const uint32_t lane = lane_id();
@@ -288,7 +284,7 @@ struct EvalFP8 {
       for (int g = 0; g < G; ++g) {
         const uint32_t idx = g0 + 16u * g + m;
         const bool ok = idx < ntodo;
-        const uint32_t cid = ok ? cand[todo[idx]] : 0u;
+        const uint32_t cid = ok ? todo[idx] : 0u;
         const long* row = reinterpret_cast<const long*>(
             base8 + size_t(cid) * dpad + 8u * kq);
 #pragma unroll
@@ -314,9 +310,8 @@ struct EvalFP8 {
           dot = m == 2u ? acc[2] : dot;
           dot = m == 3u ? acc[3] : dot;
           if (idx < ntodo) {
-            const uint32_t t = todo[idx];
-            const uint32_t cid = cand[t];
-            tail[t] = search_key(-(dot * qscale * scale8[cid]), cid);
+            const uint32_t cid = todo[idx];
+            tail[idx] = search_key(-(dot * qscale * scale8[cid]), cid);
           }
         }
       }
@@ -324,25 +319,25 @@ struct EvalFP8 {
   }
 };
 
-// Merges the sorted misses of one iteration into the top-Km list (the
-// reference re-sorts the whole list, Gsearcher.cuh:1045; only its first Km
-// entries survive into the next iteration, so a merge is equivalent).
+// Merges the misses of one iteration (keys tail[0, m)) into the top-Km list
+// (the reference re-sorts the whole list, Gsearcher.cuh:1045; only its
+// first Km entries survive into the next iteration, so a merge is
+// equivalent).
 // Returns true when this iteration is the last one (no unexpanded entry
 // would stay in the top Km, or the watchdog stops after it); the caller then
 // sorts the full list once, which also orders the tail the output needs.
 template <int R, int HPL>
 __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
-                             const uint32_t* todo, uint32_t m, uint64_t* ms,
-                             bool force_final) {
+                             uint32_t m, uint64_t* ms, bool force_final) {
   const uint32_t lane = lane_id();
   uint64_t mv[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t idx = lane * R + r;
-    mv[r] = idx < m ? tail[todo[idx]] : kPadKey;
+    mv[r] = idx < m ? tail[idx] : kPadKey;
   }
   reg_bitonic_sort<R>(mv);
-  wave_sync();  // ms reuses the candidate / todo slots read above
+  wave_sync();  // ms reuses the candidate / todo slots
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t idx = lane * R + r;
@@ -386,8 +381,8 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
   return false;
 }
 
-// Merge by ranks, for any number of entering misses.  The misses below the
-// current Km-th key T enter (keys are unique per id); they are compacted into
+// Merge by ranks, for any number of entering misses (keys tail[0, m)).  The
+// misses below the current Km-th key T enter (keys are unique per id); they are compacted into
 // ebuf, each one's position is #(entering < e) + #(kept < e) (a scan of ebuf
 // plus a binary search, one lane per entering key), and they are placed into
 // a staging copy of the top Km held in the dead tail region (it must hold Km
@@ -397,8 +392,8 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
 // unexpanded key is in the top Km, or force_final), else 0.
 template <int CH, int HPL>
 __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
-                            const uint64_t* tail, const uint32_t* todo,
-                            uint32_t m, uint64_t* ebuf, bool force_final) {
+                            const uint64_t* tail, uint32_t m, uint64_t* ebuf,
+                            bool force_final) {
   const uint32_t lane = lane_id();
   const uint64_t T = keys[Km - 1];
   uint64_t me[CH];
@@ -412,7 +407,7 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
     slot[c] = 0;
     if (64u * c < m) {
       const uint32_t idx = lane + 64u * c;
-      if (idx < m) me[c] = tail[todo[idx]];
+      if (idx < m) me[c] = tail[idx];
       in_e[c] = me[c] < T;
       const uint64_t em = __ballot(in_e[c]);
       slot[c] = ne + __popcll(em & lanemask_lt());
@@ -467,7 +462,7 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
       return 0;
     }
   }
-  wave_sync();  // ebuf overlays the todo list read above
+  wave_sync();  // ebuf overlays the candidate / todo slots
 #pragma unroll
   for (int c = 0; c < CH; ++c)
     if (in_e[c]) ebuf[slot[c]] = me[c];
@@ -571,7 +566,7 @@ __attribute__((amdgpu_waves_per_eu(kLean ? Eval::kLeanMinWaves
         ev.load_query_regs(queries, qq);
         cur_q = qq;
       }
-      ev.eval(cand, todo, nt, tail, wave, nw);
+      ev.eval(todo, nt, tail, wave, nw);
       __syncthreads();  // keys handed back
     }
     return;
@@ -621,11 +616,11 @@ __attribute__((amdgpu_waves_per_eu(kLean ? Eval::kLeanMinWaves
         hand[0] = 1u;
       }
       __syncthreads();
-      ev.eval(cand, todo, ntodo, tail, 0, nw);
+      ev.eval(todo, ntodo, tail, 0, nw);
       __syncthreads();
     } else {
       wave_sync();
-      ev.eval(cand, todo, ntodo, tail);
+      ev.eval(todo, ntodo, tail);
       wave_sync();
     }
   };
@@ -637,10 +632,8 @@ __attribute__((amdgpu_waves_per_eu(kLean ? Eval::kLeanMinWaves
   // initial list: [0, Km) = (FLT_MAX, tomb); [Km, Km+kpkd) = entry points
   for (uint32_t i = lane; i < lp2; i += kWave)
     keys[i] = i < Km ? search_key(FLT_MAX, kTomb) : kPadKey;
-  for (uint32_t t = lane; t < kpkd; t += kWave) {
-    cand[t] = uint32_t((uint64_t(entry_seed) + t) % n_base);
-    todo[t] = t;
-  }
+  for (uint32_t t = lane; t < kpkd; t += kWave)
+    todo[t] = uint32_t((uint64_t(entry_seed) + t) % n_base);
   eval_all(kpkd);
   unsigned long long ndist = kpkd;
   // The list after init is the first Km keys of sort(entry keys, Km tomb
@@ -747,7 +740,6 @@ __attribute__((amdgpu_waves_per_eu(kLean ? Eval::kLeanMinWaves
       vmiss[c] = false;
       vact[c] = t < kpkd && nid[c] != kTomb;
       vslot[c] = visit_hash(nid[c], tsize);
-      if (t < kpkd) cand[t] = nid[c];
     }
     while (true) {
       bool any = false;
@@ -768,33 +760,42 @@ __attribute__((amdgpu_waves_per_eu(kLean ? Eval::kLeanMinWaves
       }
       if (!any) break;
     }
+    // The misses' ids are compacted into todo and their keys go to
+    // tail[0, ntodo); the hits' (FLT_MAX) keys fill the rest of tail[0,
+    // kpkd): the tail holds the same keys as slot by slot, which is all the
+    // final sort sees.
     uint32_t ntodo = 0;
 #pragma unroll
     for (int c = 0; c < CH; ++c) {
-      const uint32_t t = lane + 64u * c;
       const bool miss = vmiss[c];
       // (a hit's FLT_MAX key is written only on the last iteration, below)
       const uint64_t m = __ballot(miss);
-      if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = t;
+      if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = nid[c];
       ntodo += __popcll(m);
     }
-    // candidate slots past 64*CH (Kp*Kd up to 4096), one slice at a time
+    // candidate slots past 64*CH (Kp*Kd up to 4096), one slice at a time;
+    // their hits' keys go to the end of tail[0, kpkd)
+    uint32_t nwh = 0;
     for (uint32_t c0 = 64u * CH; !kLean && c0 < kpkd; c0 += kWave) {
       const uint32_t t = c0 + lane;
       bool miss = false;
+      uint32_t id = kTomb;
       if (t < kpkd) {
         const uint32_t e = kd_shift < 32u ? (t >> kd_shift) : t / Kd;
         const uint32_t j = t - e * Kd;
         const uint32_t eid = explore[e];
-        const uint32_t id =
-            eid == kTomb ? kTomb : graph[size_t(eid) * gdeg + j];
-        cand[t] = id;
+        id = eid == kTomb ? kTomb : graph[size_t(eid) * gdeg + j];
         if (id != kTomb) miss = visit_insert(tab, tsize, id);
-        if (!miss) tail[t] = search_key(FLT_MAX, id);
       }
       const uint64_t m = __ballot(miss);
-      if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = t;
+      if (miss) todo[ntodo + __popcll(m & lanemask_lt())] = id;
       ntodo += __popcll(m);
+      const bool hit = t < kpkd && !miss;
+      const uint64_t hm = __ballot(hit);
+      if (hit)
+        tail[kpkd - 1u - (nwh + __popcll(hm & lanemask_lt()))] =
+            search_key(FLT_MAX, id);
+      nwh += __popcll(hm);
     }
     eval_all(ntodo);
     ndist += ntodo;
@@ -802,27 +803,33 @@ __attribute__((amdgpu_waves_per_eu(kLean ? Eval::kLeanMinWaves
     const bool last_allowed = total_iter + 1 >= max_iters;
     bool final_sort = true;
     if (kLean || rank_merge) {
-      final_sort = merge_ranked<CH, HPL>(keys, Km, tail, tail, todo, ntodo, ms,
+      final_sort = merge_ranked<CH, HPL>(keys, Km, tail, tail, ntodo, ms,
                                          last_allowed) == 1;
     } else if (!kLean && fast_merge) {
       if (ntodo <= 64)
-        final_sort = merge_misses<1, HPL>(keys, Km, tail, todo, ntodo, ms,
+        final_sort = merge_misses<1, HPL>(keys, Km, tail, ntodo, ms,
                                         last_allowed);
       else if (CH <= 2 || ntodo <= 128)
-        final_sort = merge_misses<(CH < 2 ? CH : 2), HPL>(keys, Km, tail, todo,
+        final_sort = merge_misses<(CH < 2 ? CH : 2), HPL>(keys, Km, tail,
                                                       ntodo, ms, last_allowed);
       else  // the register bitonic sort needs a power-of-two width
         final_sort = merge_misses<(CH <= 4 ? 4 : 8), HPL>(
-            keys, Km, tail, todo, ntodo, ms, last_allowed);
+            keys, Km, tail, ntodo, ms, last_allowed);
     }
     if (final_sort) {
       // The output's tail is the sorted union of the list and this last
       // iteration's candidates, its visited hits at FLT_MAX: their keys
-      // are written here (misses hold their distances from the evaluator).
+      // are written here, after the misses' (tail[0, ntodo)).
+      uint32_t hpos = ntodo;
 #pragma unroll
       for (int c = 0; c < CH; ++c) {
         const uint32_t t = lane + 64u * c;
-        if (t < kpkd && !vmiss[c]) tail[t] = search_key(FLT_MAX, nid[c]);
+        const bool hit = t < kpkd && !vmiss[c];
+        const uint64_t hm = __ballot(hit);
+        if (hit)
+          tail[hpos + __popcll(hm & lanemask_lt())] =
+              search_key(FLT_MAX, nid[c]);
+        hpos += __popcll(hm);
       }
       wave_sync();
       sort_list<true>(keys, lp2);
