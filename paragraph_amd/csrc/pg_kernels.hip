@@ -562,6 +562,21 @@ __device__ __forceinline__ float wave_min_f32(float v) {
   return v;
 }
 
+// Elements 8 lane .. +7 of row id (lanes < 4 NCH; zero past dim and past
+// the row's dpad / 32 chunks), as the screened sweep's B columns take them.
+template <int NCH>
+__device__ __forceinline__ void sweep_row_octet(const float* __restrict__ base,
+                                                uint32_t dim,
+                                                const ScreenArgs& sc,
+                                                uint32_t id, float (&v)[8]) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = 0.0f;
+  if (lane < 4u * NCH)
+    load_query_octet(base + size_t(id) * dim, dim, lane < 4u * (sc.dpad / 32u),
+                     8u * lane, v);
+}
+
 // LDS of one sweep (one wave).  Entries e < ne; key of e = (ordered d,
 // position << 9 | e).
 template <uint32_t DP2, int NCH, bool kScreen>
@@ -589,17 +604,20 @@ struct SweepLds {
 
 // Kept-major strict RNG over the entries set up in L (eid, epos, alive = 0 ..
 // ne-1, ef = 1 and dv = exact d for entries whose d is known, rk = 0xFFFF,
-// enr for the screened kernel), ne > 0.  x_node is the node's row, c0 the
-// speculative first kept entry.  Entries whose key is not below kstop end
-// the scan.  Returns the kept count (at most cap), kept ids in L.kept.
+// enr for the screened kernel), ne > 0.  c is the speculative first kept
+// entry.  The screened kernel takes the node's norms (nb_in, rb_in) and the
+// octets of the node's and c's FP32 rows that sweep_row_octet loaded (the
+// caller issues those loads early).  Entries whose key is not below kstop
+// end the scan.  Returns the kept count (at most cap), kept ids in L.kept.
 // pairs: exact kernel, the candidate-major comparison count; screened
 // kernel, the screened pair decisions.  fb: exact chains taken.
 template <uint32_t DP2, int NCH, bool kScreen>
 __device__ __forceinline__ uint32_t sweep_select(
     SweepLds<DP2, NCH, kScreen>& L, const float* __restrict__ base,
     uint32_t dim, uint32_t node, const ScreenArgs& sc, uint32_t ne,
-    uint32_t cap, uint64_t kstop, uint32_t c, unsigned long long& pairs_out,
-    unsigned long long& fb_out) {
+    uint32_t cap, uint64_t kstop, uint32_t c, float nb_in, float rb_in,
+    const float (&node_oct)[8], const float (&c_oct)[8],
+    unsigned long long& pairs_out, unsigned long long& fb_out) {
   constexpr int G = NCH <= 8 ? 2 : 1;            // 16-row groups in flight
   constexpr uint32_t NF = kScreen ? NCH * 4 : 1;  // fragments per row
   const uint32_t lane = lane_id();
@@ -615,12 +633,10 @@ __device__ __forceinline__ uint32_t sweep_select(
   // dim), kept in registers from its fragment load until the first sweep
   // has used the node's slots.
   float nrow[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  auto load_frag = [&](uint32_t col, uint32_t id) {
-    const float* x = base + size_t(id) * dim;
+  // column col from the octet v of lane < NF
+  auto store_frag = [&](uint32_t col, const float (&v)[8]) {
     if constexpr (kScreen) {
       if (lane < NF) {
-        float v[8];
-        load_query_octet(x, dim, lane < 4u * nch, 8u * lane, v);
         bf16x8 h, l;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -631,6 +647,13 @@ __device__ __forceinline__ uint32_t sweep_select(
         L.frag[2u * col * NF + lane] = h;
         L.frag[(2u * col + 1u) * NF + lane] = l;
       }
+    }
+  };
+  auto load_frag = [&](uint32_t col, uint32_t id) {
+    if constexpr (kScreen) {
+      float v[8];
+      sweep_row_octet<NCH>(base, dim, sc, id, v);
+      store_frag(col, v);
     }
   };
   // FP32 rows in LDS for the exact chains (screened kernel): the node's in
@@ -679,10 +702,10 @@ __device__ __forceinline__ uint32_t sweep_select(
     return pair_bound_res(sc, L.enr[e].x, L.enr[e].y, nb, rb);
   };
   if constexpr (kScreen) {
-    nb = sc.norms[node];
-    rb = sc.rnorms[node];
+    nb = nb_in;
+    rb = rb_in;
   }
-  load_frag(1, node);
+  store_frag(1, node_oct);
   auto key_of = [&](uint32_t e) -> uint64_t {
     return (uint64_t(ordered_bits(L.dv[e])) << 32) |
            (uint32_t(L.epos[e]) << 9) | e;
@@ -834,7 +857,7 @@ __device__ __forceinline__ uint32_t sweep_select(
   };
 
   uint32_t nk = 0, clast = 0;
-  load_frag(0, L.eid[c]);
+  store_frag(0, c_oct);
   wave_sync();
   sweep(c, true);
   if constexpr (kScreen) {
@@ -1004,33 +1027,71 @@ __global__ __launch_bounds__(64) void prune_sweep_kernel(
     unsigned long long* __restrict__ stats, ScreenArgs sc,
     const uint32_t* __restrict__ order) {
   __shared__ SweepLds<DP2, NCH, kScreen> L;
-  // row ids, read before the first sweep writes L.sv
-  uint32_t* sid = reinterpret_cast<uint32_t*>(L.sv);
   // nodes in the locality order when one is given (rows are per node, so
   // the launch order changes no result)
   const uint32_t b = order ? order[blockIdx.x] : blockIdx.x;
   const uint32_t lane = lane_id();
+  // The loads are issued in dependency levels, each level's loads in
+  // flight together: the node's norms and row octet with the row ids, then
+  // the entries' norms with the first kept entry's row octet.
+  float nb = 0.0f, rb = 0.0f;
+  float noct[8], coct[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) noct[j] = coct[j] = 0.0f;
+  if constexpr (kScreen) {
+    nb = sc.norms[b];
+    rb = sc.rnorms[b];
+    sweep_row_octet<NCH>(base, dim, sc, b, noct);
+  }
+  constexpr int NJ = DP2 / kWave;  // row slices of 64 ids
+  uint32_t idr[NJ];
+#pragma unroll
+  for (int r = 0; r < NJ; ++r) {
+    const uint32_t j = 64u * r + lane;
+    idr[r] = (64u * r < D && j < D) ? rows[size_t(b) * D + j] : kTomb;
+  }
   // Entries: valid candidates, and self entries before the first stop id
   // (prune_kernel sorts them at FLT_MAX by position).
   uint32_t jstop = D;
-  for (uint32_t j0 = 0; j0 < D; j0 += kWave) {
-    const uint32_t j = j0 + lane;
-    const uint32_t id = j < D ? rows[size_t(b) * D + j] : kTomb;
-    if (j < D) sid[j] = id;
-    const uint64_t sm = __ballot(j < D && is_stop_id(id, n_base));
-    if (sm && jstop == D) jstop = j0 + __ffsll((long long)sm) - 1;
+#pragma unroll
+  for (int r = NJ - 1; r >= 0; --r) {
+    const uint32_t j = 64u * r + lane;
+    const uint64_t sm = __ballot(j < D && is_stop_id(idr[r], n_base));
+    if (sm) jstop = 64u * r + __ffsll((long long)sm) - 1;
   }
-  wave_sync();
-  uint32_t ne = 0, nv = 0;
-  for (uint32_t j0 = 0; j0 < D; j0 += kWave) {
-    const uint32_t j = j0 + lane;
-    const uint32_t id = j < D ? sid[j] : kTomb;
+  // the entries' norms, and the speculative first kept entry: the first
+  // valid candidate in row order (else entry 0, a self entry)
+  float2 nrr[kScreen ? NJ : 1];
+  uint32_t cid = b, jc = D;
+#pragma unroll
+  for (int r = 0; r < NJ; ++r) {
+    const uint32_t j = 64u * r + lane;
+    const uint32_t id = idr[r];
+    const bool stop = j >= D || is_stop_id(id, n_base);
+    if constexpr (kScreen)
+      nrr[r] = stop ? make_float2(0.0f, 0.0f)
+                    : make_float2(sc.norms[id], sc.rnorms[id]);
+    const uint64_t vm = __ballot(!stop && id != b);
+    if (vm && jc == D) {
+      const int l = __ffsll((long long)vm) - 1;
+      jc = 64u * r + l;
+      cid = __builtin_amdgcn_readlane(id, l);
+    }
+  }
+  if constexpr (kScreen) sweep_row_octet<NCH>(base, dim, sc, cid, coct);
+  uint32_t ne = 0, nv = 0, c = 0xFFFFFFFFu;
+#pragma unroll
+  for (int r = 0; r < NJ; ++r) {
+    const uint32_t j = 64u * r + lane;
+    const uint32_t id = idr[r];
     const bool stop = j >= D || is_stop_id(id, n_base);
     const bool self = !stop && id == b;
     const bool valid = !stop && !self;
     const bool inc = valid || (self && j < jstop);
     const uint64_t im = __ballot(inc);
     nv += __popcll(__ballot(valid));
+    if (jc >= 64u * r && jc < 64u * r + 64u)
+      c = ne + __popcll(im & ((1ull << (jc - 64u * r)) - 1ull));
     if (inc) {
       const uint32_t e = ne + __popcll(im & lanemask_lt());
       L.eid[e] = id;
@@ -1039,8 +1100,7 @@ __global__ __launch_bounds__(64) void prune_sweep_kernel(
       L.ef[e] = self ? 1u : 0u;
       L.dv[e] = self ? FLT_MAX : 0.0f;
       L.rk[e] = 0xFFFFu;
-      if constexpr (kScreen)
-        L.enr[e] = make_float2(sc.norms[id], sc.rnorms[id]);
+      if constexpr (kScreen) L.enr[e] = nrr[r];
     }
     ne += __popcll(im);
   }
@@ -1051,16 +1111,9 @@ __global__ __launch_bounds__(64) void prune_sweep_kernel(
   uint32_t nk = 0;
   unsigned long long pairs = 0, fbk = 0;
   if (ne > 0) {
-    // speculative first kept entry: the first valid candidate in row order
-    uint32_t c = ne;
-    for (uint32_t i0 = 0; i0 < ne && c == ne; i0 += kWave) {
-      const uint32_t i = i0 + lane;
-      const uint64_t vm = __ballot(i < ne && !(L.ef[i] & 1u));
-      if (vm) c = i0 + __ffsll((long long)vm) - 1;
-    }
-    if (c == ne) c = 0;
+    if (c == 0xFFFFFFFFu) c = 0;
     nk = sweep_select<DP2, NCH, kScreen>(L, base, dim, b, sc, ne, mp, kstop,
-                                         c, pairs, fbk);
+                                         c, nb, rb, noct, coct, pairs, fbk);
   }
   for (uint32_t j = lane; j < nk; j += kWave) pruned[size_t(b) * mp + j] = L.kept[j];
   if (lane == 0) {
@@ -1309,8 +1362,25 @@ __global__ __launch_bounds__(64) void merge_kernel(
   // node, so the launch order changes no result)
   const uint32_t c = order ? order[blockIdx.x] : blockIdx.x;
   const uint32_t lane = lane_id();
-  const uint32_t s0 = off[c], seg = off[c + 1] - s0;
+  // The loads are issued in dependency levels, each level's loads in
+  // flight together: the segment bounds, the pruned count and the node's
+  // row (for the quad chains below), then the first 64 sources and pruned
+  // ids.
+  const uint32_t s0 = off[c], s1 = off[c + 1];
+  const uint32_t p = pcnt[c];
+  const float* xc = base + size_t(c) * dim;
+  const bool xrow = dim <= 2 * kSegChunk && (dim & 3u) == 0;
+  float4 xr[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const uint32_t f = lane + 64u * k;
+    xr[k] = (xrow && f < (dim >> 2))
+                ? reinterpret_cast<const float4*>(xc)[f]
+                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  const uint32_t seg = s1 - s0;
   const uint32_t r = seg < cap ? seg : cap;
+  const uint32_t pv = lane < p ? pruned[size_t(c) * mp + lane] : kTomb;
   // ---- select the r smallest sources, ascending
   if (seg <= kRankSortMax) {
     // a short segment (most nodes): each source placed at its rank
@@ -1366,14 +1436,17 @@ __global__ __launch_bounds__(64) void merge_kernel(
     // candidate list below reads only buf[0, r)
   }
   // ---- candidate list: reverse part then pruned part
-  const uint32_t p = pcnt[c];
   const uint32_t n = r + p;
+  if (lane < p) sid[r + lane] = pv;
+  for (uint32_t i = 64u + lane; i < p; i += kWave)
+    sid[r + i] = pruned[size_t(c) * mp + i];
   for (uint32_t j = lane; j < 128; j += kWave) {
-    uint32_t id = kTomb;
-    if (j < r) id = buf[j];
-    else if (j < n) id = pruned[size_t(c) * mp + (j - r)];
-    sid[j] = id;
-    if (j >= n) skey[j] = kPadKey;
+    if (j < r) {
+      sid[j] = buf[j];
+    } else if (j >= n) {
+      sid[j] = kTomb;
+      skey[j] = kPadKey;
+    }
   }
   wave_sync();
   // exact d of every candidate: up to 32 candidates as quad chains (four
@@ -1381,11 +1454,13 @@ __global__ __launch_bounds__(64) void merge_kernel(
   // staged in buf (dead once the sources are in sid); longer lists (where
   // quads would take more passes than one lane per candidate takes load
   // rounds) and wider rows take one lane per candidate
-  const float* xc = base + size_t(c) * dim;
-  if (n <= 32 && dim <= 2 * kSegChunk && (dim & 3u) == 0) {
+  if (n <= 32 && xrow) {
     float* xs = reinterpret_cast<float*>(buf);
-    for (uint32_t f = lane; f < (dim >> 2); f += kWave)
-      reinterpret_cast<float4*>(xs)[f] = reinterpret_cast<const float4*>(xc)[f];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint32_t f = lane + 64u * k;
+      if (f < (dim >> 2)) reinterpret_cast<float4*>(xs)[f] = xr[k];
+    }
     wave_sync();
     for (uint32_t b0 = 0; b0 < n; b0 += kWave / 4u) {
       const uint32_t j = b0 + (lane >> 2);
@@ -1541,6 +1616,10 @@ hipError_t launch_prune(const float* base, uint32_t n_base, uint32_t dim,
     if (sc == nullptr || sc->base16lo != nullptr) {
       if (D <= 256) PG_SWEEP(256, 0, false);
       else PG_SWEEP(512, 0, false);
+    } else if (sc->dpad == 224) {
+      // 7 chunks exactly (dims 193-224): no zero chunk in the registers
+      if (D <= 256) PG_SWEEP(256, 7, true);
+      else PG_SWEEP(512, 7, true);
     } else if (sc->dpad <= 256) {
       if (D <= 256) PG_SWEEP(256, 8, true);
       else PG_SWEEP(512, 8, true);
@@ -1642,6 +1721,8 @@ hipError_t launch_merge(const float* base, uint32_t n_base, uint32_t dim,
   } while (0)
   if (sc == nullptr || !strict) {
     PG_MERGE(0, 1, false);  // the non-strict fill runs on the plain kernel
+  } else if (sc->dpad == 224) {
+    PG_MERGE_S(7);  // 7 chunks exactly (dims 193-224)
   } else if (sc->dpad <= 256) {
     PG_MERGE_S(8);
   } else if (sc->dpad <= 512) {
