@@ -189,12 +189,15 @@ __device__ __forceinline__ float pair_bound_res(const ScreenArgs& sc, float nx,
                   sc.gam_c * nx * ny);
 }
 
+// cand_nr: when given, the (norm, residual norm) of candidate j = the low
+// word of its key, in LDS (else they are read from sc per block).
 template <bool kStopIds, int NCH, int NKT, bool kSplit>
 __device__ uint32_t rng_select_screened(
     const float* __restrict__ base, const ScreenArgs& sc, uint32_t n_base,
     uint32_t dim, const uint64_t* skey, const uint32_t* sid, uint32_t n_scan,
     uint32_t cap, uint32_t* kept, float2* kept_nr, uint32_t nk, float* gs,
-    unsigned long long* pair_total, unsigned long long* fb_total) {
+    unsigned long long* pair_total, unsigned long long* fb_total,
+    const float2* cand_nr = nullptr) {
   constexpr uint32_t GW = NKT * 16 + 16;  // gs row width
   const uint32_t lane = lane_id();
   const uint32_t m = lane & 15u, kq = lane >> 4;
@@ -204,11 +207,12 @@ __device__ uint32_t rng_select_screened(
   for (uint32_t blk = 1; blk < n_scan && !stop && nk < cap; blk += 16) {
     const uint32_t pos = blk + m;
     const bool in = pos < n_scan;
-    uint32_t my_id = kTomb;
+    uint32_t my_id = kTomb, my_j = 0;
     float my_d = FLT_MAX;
     if (in) {
       const uint64_t k = skey[pos];
-      my_id = sid[uint32_t(k)];
+      my_j = uint32_t(k);
+      my_id = sid[my_j];
       my_d = from_ordered(uint32_t(k >> 32));
     }
     uint32_t lim = 16;
@@ -320,8 +324,16 @@ __device__ uint32_t rng_select_screened(
     // ---- resolution: lane i < 16 owns block candidate i (the first
     // occluder in kept order ends a candidate's scan early)
     const bool own = lane < 16 && active;
-    const float my_norm = own ? sc.norms[my_id] : 0.0f;
-    const float my_rn = (own && sc.rnorms) ? sc.rnorms[my_id] : 0.0f;
+    float my_norm = 0.0f, my_rn = 0.0f;
+    if (own) {
+      if (cand_nr) {
+        my_norm = cand_nr[my_j].x;
+        my_rn = cand_nr[my_j].y;
+      } else {
+        my_norm = sc.norms[my_id];
+        my_rn = sc.rnorms ? sc.rnorms[my_id] : 0.0f;
+      }
+    }
     const float* own_row = base + size_t(own ? my_id : 0u) * dim;
     bool occl = false;
     uint32_t my_pairs = 0, my_fb = 0;
@@ -1356,6 +1368,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
   __shared__ uint32_t sid[128];
   __shared__ uint32_t kept[128];
   __shared__ float2 kept_nr[NCH ? 128 : 1];
+  __shared__ float2 cand_nr[NCH ? 128 : 1];  // (norm, residual) by index
   __shared__ float gs[NCH ? 16 * (NKT * 16 + 16) : 1];
   __shared__ uint8_t added[128];
   // nodes in the locality order when one is given (one output row per
@@ -1449,6 +1462,14 @@ __global__ __launch_bounds__(64) void merge_kernel(
     }
   }
   wave_sync();
+  // the screened RNG's norms of every candidate, loaded with the rows of the
+  // exact d below
+  if constexpr (NCH != 0) {
+    for (uint32_t j = lane; j < n; j += kWave) {
+      const uint32_t id = sid[j];
+      cand_nr[j] = make_float2(sc.norms[id], sc.rnorms ? sc.rnorms[id] : 0.0f);
+    }
+  }
   // exact d of every candidate: up to 32 candidates as quad chains (four
   // lanes per candidate, its whole row in flight) against the node's row,
   // staged in buf (dead once the sources are in sid); longer lists (where
@@ -1503,9 +1524,7 @@ __global__ __launch_bounds__(64) void merge_kernel(
   unsigned long long pairs = 0, fbk = 0;
   if (lane == 0) {
     kept[0] = sid[uint32_t(skey[0])];
-    if (NCH)
-      kept_nr[0] = make_float2(sc.norms[kept[0]],
-                               sc.rnorms ? sc.rnorms[kept[0]] : 0.0f);
+    if (NCH) kept_nr[0] = cand_nr[uint32_t(skey[0])];
   }
   wave_sync();
   uint32_t nk;
@@ -1513,9 +1532,9 @@ __global__ __launch_bounds__(64) void merge_kernel(
     nk = rng_select<false>(base, n_base, dim, skey, sid, n, deg, kept, 1,
                            &pairs);
   else
-    nk = rng_select_screened<false, NCH, NKT, kSplit>(base, sc, n_base, dim, skey, sid,
-                                              n, deg, kept, kept_nr, 1, gs,
-                                              &pairs, &fbk);
+    nk = rng_select_screened<false, NCH, NKT, kSplit>(
+        base, sc, n_base, dim, skey, sid, n, deg, kept, kept_nr, 1, gs,
+        &pairs, &fbk, cand_nr);
   if (!strict && nk < deg) {
     // mark the first list slot equal to each kept id, then append the
     // unmarked entries in sorted order (duplicates can be appended again).
