@@ -574,6 +574,13 @@ __device__ __forceinline__ float wave_min_f32(float v) {
   return v;
 }
 
+// float4s per lane in flight in the exact quad chains (a quad holds 4x as
+// many): the sweep's 16 take rows of up to 256 floats in one load round;
+// the merge's 13 take up to 208 (dim 200), as 16 would push the dim-512
+// split-screen merge past 256 VGPRs.
+constexpr int kSweepChainBatch = 16;
+constexpr int kMergeChainBatch = 13;
+
 // Elements 8 lane .. +7 of row id (lanes < 4 NCH; zero past dim and past
 // the row's dpad / 32 chunks), as the screened sweep's B columns take them.
 template <int NCH>
@@ -691,7 +698,7 @@ __device__ __forceinline__ uint32_t sweep_select(
       const uint32_t idx = b0 + (lane >> 2);
       const bool on = idx < n;
       const uint32_t e = on ? L.wl[idx] : 0u;
-      const float v = quad_chain<8>(xs ? xkept : xnode,
+      const float v = quad_chain<kSweepChainBatch>(xs ? xkept : xnode,
                                     base + size_t(L.eid[e]) * dim, on, dim);
       if (on && (lane & 3u) == 0) {
         ++my_fb;
@@ -1486,8 +1493,8 @@ __global__ __launch_bounds__(64) void merge_kernel(
     for (uint32_t b0 = 0; b0 < n; b0 += kWave / 4u) {
       const uint32_t j = b0 + (lane >> 2);
       const bool on = j < n;
-      const float d = quad_chain<8>(xs, base + size_t(on ? sid[j] : 0u) * dim,
-                                    on, dim);
+      const float d = quad_chain<kMergeChainBatch>(
+          xs, base + size_t(on ? sid[j] : 0u) * dim, on, dim);
       if (on && (lane & 3u) == 0)
         skey[j] = (uint64_t(ordered_bits(d)) << 32) | j;
     }
