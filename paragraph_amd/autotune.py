@@ -17,11 +17,11 @@ whose recall@10 stays within a tolerance of the FP32 build's; the FP32 path
 keeps the reference mapping.
 
 Method: one default build gives a realistic graph; then the search stage is
-timed on a sample of base rows for every search knob set, the locality
-order on all base rows (its gain is cache reuse across the whole query
-range), and the prune stage on the search rows of all base rows for the
-prune variants.  The
-fastest set wins.
+timed on a sample of base rows for every search knob set and on all base
+rows for the gather batches and grid shapes, the prune stage on the search
+rows of all base rows for the prune variants, and the locality order on
+whole builds (it orders both rounds' searches and prunes, whose graphs
+differ).  The fastest set wins.
 Results are cached in a JSON file keyed by shape, precision and device.
 """
 import json
@@ -185,27 +185,35 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             if t < best_pt:
                 best_pf, best_pt = split | scan, t
     best["tune_flags"] |= best_pf
-    # the locality order (bit 9) also orders the link-row prune: decide it
-    # on search + prune time with the chosen gather batch and prune variant
+    # The locality order (bit 9) orders the search queue and the link-row
+    # prune of both rounds, and the rounds search different graphs (the
+    # projection rows, then the link rows): on the 1M x 200 BF16 build the
+    # two orders tied on the link-round search + prune while the ordered
+    # queue made the round-1 search slower.  It is decided on whole builds
+    # with the chosen gather batch, table size and prune variant.
     if best is not None and full_t:
         g = best["tune_flags"] & 0x81F
         tot = {}
+        proj_w = torch.empty((n, p.proj_deg), **i32)
+        link_w = torch.empty((n, p.M), **i32)
         for grid in (0x20, 0x220):
             if (g | grid) not in full_t:
                 continue
             pp = _native.PgParams.from_buffer_copy(p)
-            pp.tune_flags = ((p.tune_flags & ~0xBBF) | g | grid | best_pf)
+            pp.tune_flags = ((p.tune_flags & ~0xFBF) | g | grid | best_pf)
+            pp.table_size = best["table_size"]
             with _native.Context(pp) as ctx:
-                t = _time(lambda: (ctx.run_prune_rows(base_d, full, p.M, pr,
-                                                      pc), ctx.sync()))
-            tot[grid] = full_t[g | grid] + t
+                tot[grid] = _time(lambda: (
+                    ctx.stage_device(base_d, gt_d, proj_w, link_w),
+                    ctx.wait()))
             report["prune"].append({"order": bool(grid & 0x200),
                                     "split": bool(best_pf & 0x80),
                                     "scan": bool(best_pf & 0x100),
-                                    "ms": round(1000 * t, 3)})
+                                    "ms_build": round(1000 * tot[grid], 3)})
             if log:
-                log(f"autotune order={bool(grid & 0x200)}: search + prune "
+                log(f"autotune order={bool(grid & 0x200)}: whole build "
                     f"{1000 * tot[grid]:.2f} ms")
+        del proj_w, link_w
         if tot:
             gbest = min(tot, key=tot.get)
             best["tune_flags"] = (best["tune_flags"] & ~0x220) | gbest
