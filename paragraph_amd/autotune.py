@@ -19,9 +19,10 @@ keeps the reference mapping.
 Method: one default build gives a realistic graph; then the search stage is
 timed on a sample of base rows for every search knob set and on all base
 rows for the gather batches and grid shapes, the prune stage on the search
-rows of all base rows for the prune variants, and the locality order on
-whole builds (it orders both rounds' searches and prunes, whose graphs
-differ).  The fastest set wins.
+rows of all base rows for the prune variants, and the locality order and
+visited-table size on whole builds (the order applies to both rounds'
+searches and prunes, whose graphs differ; the table sizes differ by less
+than the sample's scatter).  The fastest set wins.
 Results are cached in a JSON file keyed by shape, precision and device.
 """
 import json
@@ -190,33 +191,44 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     # projection rows, then the link rows): on the 1M x 200 BF16 build the
     # two orders tied on the link-round search + prune while the ordered
     # queue made the round-1 search slower.  It is decided on whole builds
-    # with the chosen gather batch, table size and prune variant.
+    # with the chosen gather batch and prune variant, together with the
+    # visited-table size: the sizes' search times on the sample differ by
+    # less than their scatter, while on whole 1M x 200 BF16 builds the
+    # 8x cache was 1.7% faster than the 4x one.
     if best is not None and full_t:
         g = best["tune_flags"] & 0x81F
+        tables = sorted({best["table_size"]} |
+                        {kn["table_size"] for kn in candidates(p)})
         tot = {}
         proj_w = torch.empty((n, p.proj_deg), **i32)
         link_w = torch.empty((n, p.M), **i32)
         for grid in (0x20, 0x220):
             if (g | grid) not in full_t:
                 continue
-            pp = _native.PgParams.from_buffer_copy(p)
-            pp.tune_flags = ((p.tune_flags & ~0xFBF) | g | grid | best_pf)
-            pp.table_size = best["table_size"]
-            with _native.Context(pp) as ctx:
-                tot[grid] = _time(lambda: (
-                    ctx.stage_device(base_d, gt_d, proj_w, link_w),
-                    ctx.wait()))
-            report["prune"].append({"order": bool(grid & 0x200),
-                                    "split": bool(best_pf & 0x80),
-                                    "scan": bool(best_pf & 0x100),
-                                    "ms_build": round(1000 * tot[grid], 3)})
-            if log:
-                log(f"autotune order={bool(grid & 0x200)}: whole build "
-                    f"{1000 * tot[grid]:.2f} ms")
+            for ts in tables:
+                pp = _native.PgParams.from_buffer_copy(p)
+                pp.tune_flags = ((p.tune_flags & ~0xFBF) | g | grid | best_pf)
+                pp.table_size = ts
+                if _native.check_params(pp)[0] != 0:
+                    continue
+                with _native.Context(pp) as ctx:
+                    tot[(grid, ts)] = _time(lambda: (
+                        ctx.stage_device(base_d, gt_d, proj_w, link_w),
+                        ctx.wait()))
+                report["prune"].append({"order": bool(grid & 0x200),
+                                        "table_size": ts,
+                                        "split": bool(best_pf & 0x80),
+                                        "scan": bool(best_pf & 0x100),
+                                        "ms_build": round(
+                                            1000 * tot[(grid, ts)], 3)})
+                if log:
+                    log(f"autotune order={bool(grid & 0x200)} table={ts}: "
+                        f"whole build {1000 * tot[(grid, ts)]:.2f} ms")
         del proj_w, link_w
         if tot:
-            gbest = min(tot, key=tot.get)
+            gbest, tbest = min(tot, key=tot.get)
             best["tune_flags"] = (best["tune_flags"] & ~0x220) | gbest
+            best["table_size"] = tbest
     # link-row merge in the locality order (bit 10, needs bit 9), timed on
     # the reverse lists of the pruned rows of every base row
     if best["tune_flags"] & 0x200:
