@@ -11,8 +11,9 @@ namespace pg {
 // ------------------------------------------------------------- K4: search
 // link_process_v0_store_base_data (Gsearcher.cuh:896-1066), one wave per
 // query.  LDS per wave: the visited set (open addressing, exact), the list
-// as u64 keys (distance, id, expanded bit), the candidate slots, the
-// compacted miss list and the Kp expanded ids.  Distances: half a wave (32
+// as u64 keys (distance, id, expanded bit), the compacted miss ids (todo)
+// beside a scratch region of the same size (together they hold the merge's
+// sorted or entering miss keys), and the Kp expanded ids.  Distances: half a wave (32
 // lanes) per candidate, lane l accumulating elements l, l+32, ... with the
 // query slice held in registers, then the xor tree 16/8/4/2/1, whose lane-0
 // result equals the reference's __shfl_down_sync tree.
@@ -337,7 +338,7 @@ __device__ bool merge_misses(uint64_t* keys, uint32_t Km, const uint64_t* tail,
     mv[r] = idx < m ? tail[idx] : kPadKey;
   }
   reg_bitonic_sort<R>(mv);
-  wave_sync();  // ms reuses the candidate / todo slots
+  wave_sync();  // ms overlays the todo list
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t idx = lane * R + r;
@@ -462,7 +463,7 @@ __device__ int merge_ranked(uint64_t* keys, uint32_t Km, uint64_t* stage,
       return 0;
     }
   }
-  wave_sync();  // ebuf overlays the candidate / todo slots
+  wave_sync();  // ebuf overlays the todo list
 #pragma unroll
   for (int c = 0; c < CH; ++c)
     if (in_e[c]) ebuf[slot[c]] = me[c];
@@ -543,13 +544,13 @@ __attribute__((amdgpu_waves_per_eu(kLean ? Eval::kLeanMinWaves
   uint32_t* tab = reinterpret_cast<uint32_t*>(smem + Eval::kQueryLdsBytes);
   uint64_t* keys = reinterpret_cast<uint64_t*>(
       smem + Eval::kQueryLdsBytes + ((size_t(tsize) * 4 + 15) & ~size_t(15)));
-  uint32_t* cand = reinterpret_cast<uint32_t*>(keys + lp2);
-  uint32_t* todo = cand + kpkd;
+  uint32_t* mscratch = reinterpret_cast<uint32_t*>(keys + lp2);
+  uint32_t* todo = mscratch + kpkd;
   uint32_t* explore = todo + kpkd;
   // the evaluation hand-off words: command, query, candidate count
   volatile uint32_t* hand = explore + Kp;
-  // sorted misses of the merge step: reuses cand + todo (2*kpkd words)
-  uint64_t* ms = reinterpret_cast<uint64_t*>(cand);
+  // sorted misses of the merge step: mscratch + todo (2*kpkd words)
+  uint64_t* ms = reinterpret_cast<uint64_t*>(mscratch);
   const uint32_t lane = lane_id();
   // the lean variant runs one wave per workgroup
   const uint32_t wave = kLean ? 0u : threadIdx.x >> 6;
