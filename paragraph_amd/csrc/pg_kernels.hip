@@ -698,8 +698,8 @@ __device__ __forceinline__ uint32_t sweep_select(
       const uint32_t idx = b0 + (lane >> 2);
       const bool on = idx < n;
       const uint32_t e = on ? L.wl[idx] : 0u;
-      const float v = quad_chain<kSweepChainBatch>(xs ? xkept : xnode,
-                                    base + size_t(L.eid[e]) * dim, on, dim);
+      const float v = quad_chain<kSweepChainBatch>(
+          xs ? xkept : xnode, base + size_t(L.eid[e]) * dim, on, dim);
       if (on && (lane & 3u) == 0) {
         ++my_fb;
         if (xs) {
