@@ -73,6 +73,25 @@ def candidates(p):
     return out
 
 
+def build_candidates(p, gather, prune_flags, best_table):
+    """Knob sets (PgParams copies of p) timed on whole builds: the plain
+    and the locality-ordered persistent queue, each with every
+    visited-table size (the sampled best and the candidates' sizes), at
+    the chosen gather batch / waves and prune variant; sets that fail the
+    parameter check are left out."""
+    tables = sorted({best_table} | {kn["table_size"] for kn in candidates(p)})
+    out = []
+    for grid in (0x20, 0x220):
+        for ts in tables:
+            pp = _native.PgParams.from_buffer_copy(p)
+            pp.tune_flags = ((p.tune_flags & ~0xFBF) | gather | grid |
+                             prune_flags)
+            pp.table_size = ts
+            if _native.check_params(pp)[0] == 0:
+                out.append(pp)
+    return out
+
+
 def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     """Returns ({"tune_flags", "table_size"}, report).  base_d / gt_d are
     device tensors of the workload; p is the pg_params to tune."""
@@ -197,33 +216,26 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
     # 8x cache was 1.7% faster than the 4x one.
     if best is not None and full_t:
         g = best["tune_flags"] & 0x81F
-        tables = sorted({best["table_size"]} |
-                        {kn["table_size"] for kn in candidates(p)})
         tot = {}
         proj_w = torch.empty((n, p.proj_deg), **i32)
         link_w = torch.empty((n, p.M), **i32)
-        for grid in (0x20, 0x220):
+        for pp in build_candidates(p, g, best_pf, best["table_size"]):
+            grid, ts = pp.tune_flags & 0x220, pp.table_size
             if (g | grid) not in full_t:
                 continue
-            for ts in tables:
-                pp = _native.PgParams.from_buffer_copy(p)
-                pp.tune_flags = ((p.tune_flags & ~0xFBF) | g | grid | best_pf)
-                pp.table_size = ts
-                if _native.check_params(pp)[0] != 0:
-                    continue
-                with _native.Context(pp) as ctx:
-                    tot[(grid, ts)] = _time(lambda: (
-                        ctx.stage_device(base_d, gt_d, proj_w, link_w),
-                        ctx.wait()))
-                report["prune"].append({"order": bool(grid & 0x200),
-                                        "table_size": ts,
-                                        "split": bool(best_pf & 0x80),
-                                        "scan": bool(best_pf & 0x100),
-                                        "ms_build": round(
-                                            1000 * tot[(grid, ts)], 3)})
-                if log:
-                    log(f"autotune order={bool(grid & 0x200)} table={ts}: "
-                        f"whole build {1000 * tot[(grid, ts)]:.2f} ms")
+            with _native.Context(pp) as ctx:
+                tot[(grid, ts)] = _time(lambda: (
+                    ctx.stage_device(base_d, gt_d, proj_w, link_w),
+                    ctx.wait()))
+            report["prune"].append({"order": bool(grid & 0x200),
+                                    "table_size": ts,
+                                    "split": bool(best_pf & 0x80),
+                                    "scan": bool(best_pf & 0x100),
+                                    "ms_build": round(
+                                        1000 * tot[(grid, ts)], 3)})
+            if log:
+                log(f"autotune order={bool(grid & 0x200)} table={ts}: "
+                    f"whole build {1000 * tot[(grid, ts)]:.2f} ms")
         del proj_w, link_w
         if tot:
             gbest, tbest = min(tot, key=tot.get)

@@ -106,6 +106,32 @@ def test_autotune_candidates_valid():
         assert pp.tune_flags == 0x21
 
 
+def test_autotune_whole_build_candidates():
+    """The knob sets the tuner times on whole builds: both queue orders
+    with every visited-table size it knows for the precision (the sampled
+    best included), the chosen gather batch and prune variant kept, only
+    perf knobs changed, every set valid."""
+    from paragraph_amd import autotune
+    for prec, gather in ((_native.PG_FP32, 4), (_native.PG_BF16, 1),
+                         (_native.PG_FP8E4M3, 0x801)):
+        p = _native.make_params(100000, 200, 100000, 32, 48, precision=prec)
+        sampled = {kn["table_size"] for kn in autotune.candidates(p)}
+        for best_table in sorted(sampled):
+            sets = autotune.build_candidates(p, gather, 0x80, best_table)
+            got = {(pp.tune_flags & 0x220, pp.table_size) for pp in sets}
+            assert got == {(g, t) for g in (0x20, 0x220) for t in sampled}
+            for pp in sets:
+                assert _native.check_params(pp)[0] == 0
+                assert pp.tune_flags & 0x81F == gather
+                assert pp.tune_flags & 0x80
+                for f in ("M", "Km", "Kp", "Kd", "topk", "rerank",
+                          "precision", "rounds", "reset_iter", "entry_seed"):
+                    assert getattr(pp, f) == getattr(p, f)
+        # FP32 keeps the exact visited set: one (automatic) table size
+        if prec == _native.PG_FP32:
+            assert sampled == {0}
+
+
 def test_builder_host_stages_choice():
     """BuildIndex runs the host stages on the GPU only when the gt has one
     row per base row; "auto" falls back to the host stages otherwise."""
