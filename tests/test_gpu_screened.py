@@ -188,7 +188,7 @@ def test_screened_prune_and_merge_equal_oracle(scan, split, integer, dim, mp,
 def test_screened_whole_stage_matches_oracle_given_its_search(cuda):
     """In the screened build only the search differs from the FP32 path:
     running the oracle's prune / merge on the screened search rows gives the
-    screened link graph exactly (round 1)."""
+    screened link graph exactly (a build of one link round)."""
     import torch
     from paragraph_amd import datagen
     n, dim, M, L = 1500, 200, 32, 64
