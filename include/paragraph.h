@@ -88,8 +88,10 @@ typedef struct pg_params {
                            counter; bit 6 (0x40): capture the pipeline in a
                            hipGraph once per buffer set and replay it (no
                            per-stage times in that mode); bit 7 (0x80):
-                           split-BF16 screen in the screened prune / merge
-                           (tighter bound, fewer exact fallbacks); bit 8
+                           split-BF16 screen in the screened merge and
+                           candidate-major prune (tighter bound, fewer
+                           exact fallbacks; the kept-major sweep keeps the
+                           plain screen); bit 8
                            (0x100): prune the link rows candidate by
                            candidate (the reference's loop order) instead of
                            kept-major sweeps (same rows); bit 9 (0x200):

@@ -153,9 +153,10 @@ uint32_t effective_table_size(const pg_params& p) {
 //             BF16: 16-candidate groups in flight, 1/2)
 //   bit 5     persistent search grid fed by a device work counter
 //   bit 6     capture the pipeline in a hipGraph once, replay it per build
-//   bit 7     split-BF16 screen in prune / merge (x = hi + lo, tighter
-//             bound, twice the screen bytes); the link-row prune then runs
-//             its sweep on exact FP32 chains
+//   bit 7     split-BF16 screen in the merge and the candidate-major prune
+//             (x = hi + lo, tighter bound, twice the screen bytes); the
+//             kept-major sweep screens with the plain bound in every
+//             screened context
 //   bit 8     link-row prune candidate by candidate (the reference's loop
 //             order) instead of the kept-major sweep
 //   bit 9     persistent search takes its queries in a locality order
