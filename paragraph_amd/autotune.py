@@ -133,6 +133,14 @@ def autotune(base_d, gt_d, p, sample=20000, use_cache=True, log=None):
             log(f"autotune search {kn}: {1000 * t:.2f} ms")
         if t < best_t:
             best, best_t = dict(kn), t
+    if best is None:
+        # no sampled set passed the parameter check: keep p's own knobs
+        # (valid, since its context was created above) and tune the rest
+        best = {"tune_flags": int(p.tune_flags & 0xFBF),
+                "table_size": int(p.table_size)}
+        if log:
+            log("autotune: no search knob set passed the parameter check; "
+                "keeping the given ones")
     # gather batch and grid shape on all base rows, with the best sampled
     # visited-table size: the persistent grid, and the persistent grid
     # taking its queries in the locality order (cache reuse and load
